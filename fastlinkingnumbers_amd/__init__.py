@@ -1,0 +1,318 @@
+"""MI355X linking-number engine: Python binding of the C ABI in include/flk.h.
+
+The product path is libflk.so (hand-written HIP kernels for gfx950) plus the
+C++ drop-in host library; this module only loads them with ctypes and converts
+numpy arrays.  Loading fails loudly when the built library is missing: there is
+no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(_HERE, "lib")
+LIBFLK_PATH = os.path.join(LIB_DIR, "libflk.so")
+LIBVC_PATH = os.path.join(LIB_DIR, "libverifycurves.so")
+VERIFYCURVES_BIN = os.path.join(LIB_DIR, "verifycurves")
+OBJ2BCC_BIN = os.path.join(LIB_DIR, "obj2bcc")
+
+FLK_DIRECTSUM = 0
+FLK_BARNESHUT = 1
+FLK_CRIT_BETA = 0
+FLK_CRIT_MLP = 1
+FLK_PAIR_FALLBACK = 1
+FLK_PAIR_NAN = 2
+FLK_PAIR_RETRY = 4
+FLK_PAIR_DIRECT = 8
+MLP_NPARAM = 5057
+MLP_NFEAT = 12
+
+
+class FlkParams(C.Structure):
+    _fields_ = [
+        ("method", C.c_int32),
+        ("criterion", C.c_int32),
+        ("beta_init", C.c_double),
+        ("beta_limit", C.c_double),
+        ("eps_sq", C.c_double),
+        ("target_error", C.c_double),
+        ("err_scale", C.c_double),
+        ("fallback_tol", C.c_double),
+        ("bh_direct_threshold", C.c_int64),
+        ("leaf_size", C.c_int32),
+        ("fp64", C.c_int32),
+        ("mlp_log_tol", C.c_double),
+        ("reserved", C.c_int32 * 8),
+    ]
+
+
+class FlkStats(C.Structure):
+    _fields_ = [
+        ("ms_total", C.c_double),
+        ("ms_kernel", C.c_double),
+        ("ms_ds_kernel", C.c_double),
+        ("ms_bh_traverse", C.c_double),
+        ("ms_bh_build", C.c_double),
+        ("ms_fallback", C.c_double),
+        ("seg_pairs", C.c_int64),
+        ("ds_seg_pairs", C.c_int64),
+        ("bh_visits", C.c_int64),
+        ("bh_far", C.c_int64),
+        ("bh_leaf_pairs", C.c_int64),
+        ("fallbacks", C.c_int64),
+        ("retries", C.c_int64),
+        ("ds_launches", C.c_int32),
+        ("ds_waves", C.c_int32),
+        ("reserved", C.c_int64 * 8),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+# Every function declared in include/flk.h: (name, restype, argtypes).
+_P = C.POINTER
+FLK_EXPORTS = [
+    ("flk_default_params", C.c_int32, [_P(FlkParams)]),
+    ("flk_create", C.c_int32, [C.c_int32, _P(C.c_void_p)]),
+    ("flk_destroy", None, [C.c_void_p]),
+    ("flk_last_error", C.c_char_p, [C.c_void_p]),
+    ("flk_device_name", C.c_int32, [C.c_void_p, C.c_char_p, C.c_int32]),
+    ("flk_set_curves", C.c_int32, [C.c_void_p, _P(C.c_double), _P(C.c_int64), C.c_int32]),
+    ("flk_curve_boxes", C.c_int32, [C.c_void_p, _P(C.c_double)]),
+    ("flk_potential_links", C.c_int32,
+     [C.c_void_p, _P(C.c_double), C.c_int32, _P(_P(C.c_int32)), _P(C.c_int64)]),
+    ("flk_linking_numbers", C.c_int32,
+     [C.c_void_p, _P(C.c_int32), C.c_int64, _P(FlkParams), _P(C.c_double), _P(C.c_double),
+      _P(C.c_uint32)]),
+    ("flk_get_stats", C.c_int32, [C.c_void_p, _P(FlkStats)]),
+    ("flk_mlp_set_weights", C.c_int32, [C.c_void_p, _P(C.c_float), _P(C.c_float)]),
+    ("flk_mlp_get_weights", C.c_int32, [C.c_void_p, _P(C.c_float), _P(C.c_float)]),
+    ("flk_mlp_forward", C.c_int32, [C.c_void_p, _P(C.c_float), C.c_int64, _P(C.c_float)]),
+    ("flk_mlp_gradient", C.c_int32,
+     [C.c_void_p, _P(C.c_float), _P(C.c_float), C.c_int64, _P(C.c_float), _P(C.c_float)]),
+    ("flk_mlp_train", C.c_int32,
+     [C.c_void_p, _P(C.c_float), _P(C.c_float), C.c_int64, C.c_int32, C.c_int32, C.c_float,
+      C.c_uint64, _P(C.c_float)]),
+    ("flk_mlp_samples", C.c_int32,
+     [C.c_void_p, C.c_int32, C.c_int32, C.c_int64, C.c_uint64, _P(C.c_float), _P(C.c_float),
+      _P(C.c_int64)]),
+]
+
+_lib = None
+
+
+def load_library(path: str = LIBFLK_PATH) -> C.CDLL:
+    """Loads libflk.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} is missing: build the HIP engine first (python -c 'import __graft_entry__ as g; g.build()')")
+    lib = C.CDLL(path)
+    for name, res, args in FLK_EXPORTS:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class FlkError(RuntimeError):
+    pass
+
+
+def _ptr(a: np.ndarray | None, ctype):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def default_params(**kw) -> FlkParams:
+    lib = load_library()
+    p = FlkParams()
+    lib.flk_default_params(C.byref(p))
+    for k, v in kw.items():
+        if k == "method" and isinstance(v, str):
+            v = {"directsum": FLK_DIRECTSUM, "barneshut": FLK_BARNESHUT}[v]
+        if k == "criterion" and isinstance(v, str):
+            v = {"beta": FLK_CRIT_BETA, "mlp": FLK_CRIT_MLP}[v]
+        setattr(p, k, v)
+    return p
+
+
+def pack_curves(curves: Sequence[np.ndarray]) -> tuple[np.ndarray, np.ndarray]:
+    """List of (n_i, 3) arrays -> (xyz AoS float64, offsets int64)."""
+    offs = np.zeros(len(curves) + 1, dtype=np.int64)
+    for i, c in enumerate(curves):
+        offs[i + 1] = offs[i] + len(c)
+    xyz = np.zeros((int(offs[-1]), 3), dtype=np.float64)
+    for i, c in enumerate(curves):
+        xyz[offs[i]:offs[i + 1]] = np.asarray(c, dtype=np.float64).reshape(-1, 3)
+    return np.ascontiguousarray(xyz.reshape(-1)), offs
+
+
+@dataclass
+class LinkResult:
+    lk: np.ndarray
+    err: np.ndarray
+    flags: np.ndarray
+    stats: dict = field(default_factory=dict)
+
+
+class Engine:
+    """One engine context = one HIP stream on one device (include/flk.h)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        rc = self.lib.flk_create(device, C.byref(h))
+        if rc != 0:
+            raise FlkError(f"flk_create failed ({rc}): {self.lib.flk_last_error(None).decode()}")
+        self.h = h
+        self.ncurves = 0
+        self.sizes = np.zeros(0, dtype=np.int64)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.flk_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc: int, what: str):
+        if rc != 0:
+            raise FlkError(f"{what} failed ({rc}): {self.lib.flk_last_error(self.h).decode()}")
+
+    def device_name(self) -> str:
+        buf = C.create_string_buffer(256)
+        self._check(self.lib.flk_device_name(self.h, buf, 256), "flk_device_name")
+        return buf.value.decode()
+
+    def set_curves(self, curves: Sequence[np.ndarray]):
+        xyz, offs = pack_curves(curves)
+        self.set_curves_packed(xyz, offs)
+
+    def set_curves_packed(self, xyz: np.ndarray, offs: np.ndarray):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        offs = np.ascontiguousarray(offs, dtype=np.int64)
+        n = len(offs) - 1
+        self._check(self.lib.flk_set_curves(self.h, _ptr(xyz, C.c_double), _ptr(offs, C.c_int64), n),
+                    "flk_set_curves")
+        self.ncurves = n
+        self.sizes = np.diff(offs)
+
+    def curve_boxes(self) -> np.ndarray:
+        out = np.zeros((self.ncurves, 6), dtype=np.float64)
+        self._check(self.lib.flk_curve_boxes(self.h, _ptr(out, C.c_double)), "flk_curve_boxes")
+        return out
+
+    def potential_links(self, boxes: np.ndarray | None = None) -> np.ndarray:
+        p = C.POINTER(C.c_int32)()
+        n = C.c_int64()
+        if boxes is None:
+            rc = self.lib.flk_potential_links(self.h, None, self.ncurves, C.byref(p), C.byref(n))
+        else:
+            b = np.ascontiguousarray(boxes, dtype=np.float64)
+            rc = self.lib.flk_potential_links(self.h, _ptr(b, C.c_double), len(b), C.byref(p),
+                                              C.byref(n))
+        self._check(rc, "flk_potential_links")
+        if n.value == 0:
+            return np.zeros((0, 2), dtype=np.int32)
+        return np.ctypeslib.as_array(p, shape=(n.value * 2,)).reshape(-1, 2).copy()
+
+    def linking_numbers(self, pairs: np.ndarray, params: FlkParams | None = None, **kw) -> LinkResult:
+        pairs = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1, 2))
+        m = len(pairs)
+        if params is None:
+            params = default_params(**kw)
+        lk = np.zeros(m, dtype=np.float64)
+        err = np.zeros(m, dtype=np.float64)
+        flags = np.zeros(m, dtype=np.uint32)
+        rc = self.lib.flk_linking_numbers(self.h, _ptr(pairs, C.c_int32), m, C.byref(params),
+                                          _ptr(lk, C.c_double), _ptr(err, C.c_double),
+                                          _ptr(flags, C.c_uint32))
+        self._check(rc, "flk_linking_numbers")
+        return LinkResult(lk, err, flags, self.stats())
+
+    def stats(self) -> dict:
+        s = FlkStats()
+        self._check(self.lib.flk_get_stats(self.h, C.byref(s)), "flk_get_stats")
+        return s.as_dict()
+
+    # ---- learned criterion
+    def mlp_set_weights(self, params: np.ndarray, norm: np.ndarray):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        nm = np.ascontiguousarray(norm, dtype=np.float32)
+        assert p.size == MLP_NPARAM and nm.size == 2 * MLP_NFEAT
+        self._check(self.lib.flk_mlp_set_weights(self.h, _ptr(p, C.c_float), _ptr(nm, C.c_float)),
+                    "flk_mlp_set_weights")
+
+    def mlp_get_weights(self) -> tuple[np.ndarray, np.ndarray]:
+        p = np.zeros(MLP_NPARAM, dtype=np.float32)
+        nm = np.zeros(2 * MLP_NFEAT, dtype=np.float32)
+        self._check(self.lib.flk_mlp_get_weights(self.h, _ptr(p, C.c_float), _ptr(nm, C.c_float)),
+                    "flk_mlp_get_weights")
+        return p, nm
+
+    def mlp_forward(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, MLP_NFEAT)
+        y = np.zeros(len(x), dtype=np.float32)
+        self._check(self.lib.flk_mlp_forward(self.h, _ptr(x, C.c_float), len(x), _ptr(y, C.c_float)),
+                    "flk_mlp_forward")
+        return y
+
+    def mlp_gradient(self, x: np.ndarray, y: np.ndarray) -> tuple[np.ndarray, float]:
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, MLP_NFEAT)
+        y = np.ascontiguousarray(y, dtype=np.float32).reshape(-1)
+        g = np.zeros(MLP_NPARAM, dtype=np.float32)
+        loss = C.c_float()
+        self._check(self.lib.flk_mlp_gradient(self.h, _ptr(x, C.c_float), _ptr(y, C.c_float), len(x),
+                                              _ptr(g, C.c_float), C.byref(loss)), "flk_mlp_gradient")
+        return g, loss.value
+
+    def mlp_train(self, x: np.ndarray, y: np.ndarray, epochs: int = 10, batch: int = 4096,
+                  lr: float = 1e-3, seed: int = 5) -> float:
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, MLP_NFEAT)
+        y = np.ascontiguousarray(y, dtype=np.float32).reshape(-1)
+        loss = C.c_float()
+        self._check(self.lib.flk_mlp_train(self.h, _ptr(x, C.c_float), _ptr(y, C.c_float), len(x),
+                                           epochs, batch, lr, seed, C.byref(loss)), "flk_mlp_train")
+        return loss.value
+
+    def mlp_samples(self, a: int, b: int, cap: int, seed: int = 5) -> tuple[np.ndarray, np.ndarray]:
+        x = np.zeros((cap, MLP_NFEAT), dtype=np.float32)
+        y = np.zeros(cap, dtype=np.float32)
+        n = C.c_int64()
+        self._check(self.lib.flk_mlp_samples(self.h, a, b, cap, seed, _ptr(x, C.c_float),
+                                             _ptr(y, C.c_float), C.byref(n)), "flk_mlp_samples")
+        return x[: n.value].copy(), y[: n.value].copy()
+
+
+def certificate_from_lk(pairs: np.ndarray, lk: np.ndarray) -> list[tuple[int, int, int]]:
+    """Rounds pair sums into certificate triplets as the reference does
+    (src/linking_number_certificate.cpp:185-213, :294-322): nearest integer
+    (half away from zero), kept when |round| >= 0.25, sorted by (i, j)."""
+    out = []
+    for (i, j), v in zip(np.asarray(pairs).reshape(-1, 2), lk):
+        r = np.sign(v) * np.floor(abs(v) + 0.5) if np.isfinite(v) else v
+        if not (abs(r) < 0.25):
+            out.append((int(min(i, j)), int(max(i, j)), int(r) if np.isfinite(r) else -2 ** 31))
+    out.sort()
+    return out

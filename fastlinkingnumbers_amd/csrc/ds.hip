@@ -1,0 +1,453 @@
+// Direct-summation tile kernels (gfx950).
+//
+// The Gauss linking integral of two closed polylines is the sum over segment
+// pairs of the signed solid angle of the quadrilateral (a, b, c, d) spanned by
+// the relative vectors of their endpoints, split into the two triangles
+// (a, b, c) and (a, d, c) with the Van Oosterom–Strackee tangent formula and
+// recombined into one arctangent (Arai 2013; reference
+// src/compute_link.cpp:63-84 and :151-175).
+//
+// Work layout.  A wave owns 64*R consecutive rows (segments of the longer
+// curve, R per lane, kept in VGPRs) and sweeps a range of columns (segments of
+// the other curve).  The column vertices are wave-uniform, so they come
+// through the scalar cache into SGPRs: no LDS traffic, no bank conflicts.  The
+// grid of relative vectors G(i, j) = q_j - p_i is shared between the four
+// segment pairs that touch it, so each pair costs one new grid point (one
+// sqrt), one row dot, one column dot, one diagonal dot, the triple product
+// and one arctangent.  Per-lane sums are FP32 over short column runs and FP64
+// beyond; the integer branch corrections are exact.
+//
+// Load balance.  All tasks of all pairs form one list with an integer cost
+// prefix; each wave of the persistent grid takes the tasks whose starting cost
+// falls in its equal share.  Every task writes its own partial, and a second
+// kernel reduces the partials of each pair in a fixed order, so results are
+// deterministic.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+
+#include "flk_internal.h"
+
+namespace flk {
+
+namespace {
+
+constexpr float kPiF = 3.14159265358979323846f;
+constexpr float kHalfPiF = 1.57079632679489661923f;
+
+// atan(t) = t * P(t^2) on [0, 1]; coefficients fitted for this engine by a
+// relative-error minimax (Lawson) iteration against mpmath; max relative error
+// 1e-7 in exact arithmetic, 2.4e-7 evaluated in FP32.
+__device__ __forceinline__ float atan_unit(float t) {
+  const float s = t * t;
+  float p = -0.004693266935646534f;
+  p = fmaf(p, s, 0.024252379313111305f);
+  p = fmaf(p, s, -0.05948637053370476f);
+  p = fmaf(p, s, 0.09914292395114899f);
+  p = fmaf(p, s, -0.1401948183774948f);
+  p = fmaf(p, s, 0.19969724118709564f);
+  p = fmaf(p, s, -0.33331990242004395f);
+  p = fmaf(p, s, 0.9999998807907104f);
+  return t * p;
+}
+
+// Four-quadrant arctangent with IEEE atan2 sign conventions for signed zeros.
+__device__ __forceinline__ float atan2_fast(float y, float x) {
+  const float ax = fabsf(x), ay = fabsf(y);
+  const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+  const float t = mn * __builtin_amdgcn_rcpf(fmaxf(mx, 1e-37f));
+  float r = atan_unit(t);
+  r = (ay > ax) ? (kHalfPiF - r) : r;
+  r = (x < 0.f) ? (kPiF - r) : r;
+  return copysignf(r, y);
+}
+
+__device__ __forceinline__ int64_t lower_bound_i64(const int64_t *__restrict__ a, int64_t n,
+                                                   int64_t key) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (a[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <typename T> __device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// ---------------------------------------------------------------- FP32 tiles
+template <int R, bool PRECISE>
+__global__ __launch_bounds__(256) void ds_tile_f32(
+    const float4 *__restrict__ hi, const float4 *__restrict__ lo,
+    const float4 *__restrict__ ed, const DsPair *__restrict__ pairs,
+    const DsTask *__restrict__ tasks, const int64_t *__restrict__ cost_prefix,
+    int64_t ntasks, double *__restrict__ task_theta, int32_t *__restrict__ task_corr) {
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+  const int nwaves = (int)((gridDim.x * blockDim.x) >> 6);
+  const int64_t total = cost_prefix[ntasks];
+  const int64_t tb = lower_bound_i64(cost_prefix, ntasks, total / nwaves * wave +
+                                                              (total % nwaves) * wave / nwaves);
+  const int64_t te = lower_bound_i64(cost_prefix, ntasks, total / nwaves * (wave + 1) +
+                                                              (total % nwaves) * (wave + 1) / nwaves);
+
+  for (int64_t t = tb; t < te; ++t) {
+    const DsTask tk = tasks[t];
+    const DsPair pr = pairs[tk.pair];
+    const int rbase = tk.row0 + lane * R;
+
+    float px[R + 1], py[R + 1], pz[R + 1];
+    float lx[R + 1], ly[R + 1], lz[R + 1];
+    float ex[R], ey[R], ez[R];
+    bool valid[R];
+#pragma unroll
+    for (int k = 0; k <= R; ++k) {
+      const int64_t v = pr.off_r + min(rbase + k, pr.n_r);
+      const float4 h = hi[v];
+      px[k] = h.x; py[k] = h.y; pz[k] = h.z;
+      if (PRECISE) {
+        const float4 l = lo[v];
+        lx[k] = l.x; ly[k] = l.y; lz[k] = l.z;
+      } else {
+        lx[k] = ly[k] = lz[k] = 0.f;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int64_t v = pr.off_r + min(rbase + k, pr.n_r);
+      const float4 e = ed[v];
+      ex[k] = e.x; ey[k] = e.y; ez[k] = e.z;
+      valid[k] = rbase + k < pr.n_r;
+    }
+
+    // Relative vectors of one grid column: G(k, j) = q_j - p_k, its norm, and
+    // the column dot G(k, j) . G(k+1, j).
+    auto column = [&](int64_t v, float (&gx)[R + 1], float (&gy)[R + 1], float (&gz)[R + 1],
+                      float (&gl)[R + 1]) {
+      const float4 q = hi[v];
+#pragma unroll
+      for (int k = 0; k <= R; ++k) {
+        if constexpr (PRECISE) {
+          const float4 ql = lo[v];
+          gx[k] = (q.x - px[k]) + (ql.x - lx[k]);
+          gy[k] = (q.y - py[k]) + (ql.y - ly[k]);
+          gz[k] = (q.z - pz[k]) + (ql.z - lz[k]);
+        } else {
+          gx[k] = q.x - px[k];
+          gy[k] = q.y - py[k];
+          gz[k] = q.z - pz[k];
+        }
+        gl[k] = __builtin_amdgcn_sqrtf(fmaf(gx[k], gx[k], fmaf(gy[k], gy[k], gz[k] * gz[k])));
+      }
+    };
+    float ax[R + 1], ay[R + 1], az[R + 1], al[R + 1], acol[R];
+    float bx[R + 1], by[R + 1], bz[R + 1], bl[R + 1], bcol[R];
+    column(pr.off_c + tk.col0, ax, ay, az, al);
+#pragma unroll
+    for (int k = 0; k < R; ++k)
+      acol[k] = fmaf(ax[k], ax[k + 1], fmaf(ay[k], ay[k + 1], az[k] * az[k + 1]));
+
+    // One column step: the new column (j+1) from q_{j+1}, then the R segment
+    // pairs (k, j) of the quadrilaterals a=G(k,j) b=G(k+1,j) c=G(k+1,j+1)
+    // d=G(k,j+1).
+    auto step = [&](int64_t v, const float (&gx)[R + 1], const float (&gy)[R + 1],
+                    const float (&gz)[R + 1], const float (&gl)[R + 1], const float (&gcol)[R],
+                    float (&nx)[R + 1], float (&ny)[R + 1], float (&nz)[R + 1],
+                    float (&nl)[R + 1], float (&ncol)[R], float &accf, int &corr) {
+      const float4 e2 = ed[v];
+      column(v + 1, nx, ny, nz, nl);
+      float row[R + 1], diag[R];
+#pragma unroll
+      for (int k = 0; k <= R; ++k) row[k] = fmaf(gx[k], nx[k], fmaf(gy[k], ny[k], gz[k] * nz[k]));
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        ncol[k] = fmaf(nx[k], nx[k + 1], fmaf(ny[k], ny[k + 1], nz[k] * nz[k + 1]));
+        diag[k] = fmaf(gx[k], nx[k + 1], fmaf(gy[k], ny[k + 1], gz[k] * nz[k + 1]));
+      }
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const float la = gl[k], lb = gl[k + 1], lc = nl[k + 1], ld = nl[k];
+        const float s = fmaf(la, lc, diag[k]);
+        const float div1 = fmaf(lb, s, fmaf(gcol[k], lc, row[k + 1] * la));
+        const float div2 = fmaf(ld, s, fmaf(row[k], lc, ncol[k] * la));
+        // det = a . (e2 x e1)
+        const float cx = fmaf(e2.y, ez[k], -e2.z * ey[k]);
+        const float cy = fmaf(e2.z, ex[k], -e2.x * ez[k]);
+        const float cz = fmaf(e2.x, ey[k], -e2.y * ex[k]);
+        const float det = fmaf(gx[k], cx, fmaf(gy[k], cy, gz[k] * cz));
+        const float xp = fmaf(div1, div2, -det * det);
+        const float s12 = div1 + div2;
+        const float yp = det * s12;
+        const float th = atan2_fast(yp, xp);
+        const int cr = (s12 < 0.f && det != 0.f) ? (det > 0.f ? 1 : -1) : 0;
+        accf += valid[k] ? th : 0.f;
+        corr += valid[k] ? cr : 0;
+      }
+    };
+
+    double acc = 0.0;
+    int corr = 0;
+    const int jend = tk.col0 + tk.ncols;
+    int j = tk.col0;
+    while (j < jend) {
+      const int jrun = min(jend, j + 64);
+      float accf = 0.f;
+      for (; j + 1 < jrun; j += 2) {
+        step(pr.off_c + j, ax, ay, az, al, acol, bx, by, bz, bl, bcol, accf, corr);
+        step(pr.off_c + j + 1, bx, by, bz, bl, bcol, ax, ay, az, al, acol, accf, corr);
+      }
+      if (j < jrun) {
+        step(pr.off_c + j, ax, ay, az, al, acol, bx, by, bz, bl, bcol, accf, corr);
+#pragma unroll
+        for (int k = 0; k <= R; ++k) {
+          ax[k] = bx[k]; ay[k] = by[k]; az[k] = bz[k]; al[k] = bl[k];
+        }
+#pragma unroll
+        for (int k = 0; k < R; ++k) acol[k] = bcol[k];
+        ++j;
+      }
+      acc += (double)accf;
+    }
+    acc = wave_sum(acc);
+    corr = wave_sum(corr);
+    if (lane == 0) {
+      task_theta[t] = acc;
+      task_corr[t] = corr;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- FP64 tiles
+// The judge: same tiling, FP64 coordinates, library atan2 and sqrt.
+template <int R>
+__global__ __launch_bounds__(256) void ds_tile_f64(
+    const double4 *__restrict__ pts, const DsPair *__restrict__ pairs,
+    const DsTask *__restrict__ tasks, const int64_t *__restrict__ cost_prefix, int64_t ntasks,
+    double *__restrict__ task_theta, int32_t *__restrict__ task_corr) {
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+  const int nwaves = (int)((gridDim.x * blockDim.x) >> 6);
+  const int64_t total = cost_prefix[ntasks];
+  const int64_t tb = lower_bound_i64(cost_prefix, ntasks, total / nwaves * wave +
+                                                              (total % nwaves) * wave / nwaves);
+  const int64_t te = lower_bound_i64(cost_prefix, ntasks, total / nwaves * (wave + 1) +
+                                                              (total % nwaves) * (wave + 1) / nwaves);
+  for (int64_t t = tb; t < te; ++t) {
+    const DsTask tk = tasks[t];
+    const DsPair pr = pairs[tk.pair];
+    const int rbase = tk.row0 + lane * R;
+    double px[R + 1], py[R + 1], pz[R + 1];
+    bool valid[R];
+#pragma unroll
+    for (int k = 0; k <= R; ++k) {
+      const double4 h = pts[pr.off_r + min(rbase + k, pr.n_r)];
+      px[k] = h.x; py[k] = h.y; pz[k] = h.z;
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) valid[k] = rbase + k < pr.n_r;
+    double acc = 0.0;
+    int corr = 0;
+    for (int j = tk.col0; j < tk.col0 + tk.ncols; ++j) {
+      const double4 q0 = pts[pr.off_c + j];
+      const double4 q1 = pts[pr.off_c + j + 1];
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const double ax = q0.x - px[k], ay = q0.y - py[k], az = q0.z - pz[k];
+        const double bx = q0.x - px[k + 1], by = q0.y - py[k + 1], bz = q0.z - pz[k + 1];
+        const double cx = q1.x - px[k + 1], cy = q1.y - py[k + 1], cz = q1.z - pz[k + 1];
+        const double dx = q1.x - px[k], dy = q1.y - py[k], dz = q1.z - pz[k];
+        const double la = sqrt(ax * ax + ay * ay + az * az);
+        const double lb = sqrt(bx * bx + by * by + bz * bz);
+        const double lc = sqrt(cx * cx + cy * cy + cz * cz);
+        const double ld = sqrt(dx * dx + dy * dy + dz * dz);
+        const double ac = cx * ax + cy * ay + cz * az;
+        const double div1 = la * lb * lc + (ax * bx + ay * by + az * bz) * lc + ac * lb +
+                            (cx * bx + cy * by + cz * bz) * la;
+        const double div2 = la * ld * lc + (ax * dx + ay * dy + az * dz) * lc + ac * ld +
+                            (cx * dx + cy * dy + cz * dz) * la;
+        const double det = ax * (by * cz - bz * cy) + ay * (bz * cx - bx * cz) +
+                           az * (bx * cy - by * cx);
+        const double xp = div1 * div2 - det * det;
+        const double yp = det * (div1 + div2);
+        const double th = atan2(yp, xp);
+        const int cr = (yp * det < 0.0) ? (det > 0.0 ? 1 : -1) : 0;
+        acc += valid[k] ? th : 0.0;
+        corr += valid[k] ? cr : 0;
+      }
+    }
+    acc = wave_sum(acc);
+    corr = wave_sum(corr);
+    if (lane == 0) {
+      task_theta[t] = acc;
+      task_corr[t] = corr;
+    }
+  }
+}
+
+// One wave per pair: fixed-order reduction of the pair's task partials.
+__global__ __launch_bounds__(256) void ds_pair_reduce(const double *__restrict__ task_theta,
+                                                      const int32_t *__restrict__ task_corr,
+                                                      const int64_t *__restrict__ pair_task0,
+                                                      int64_t npairs, double *__restrict__ lk) {
+  const int lane = threadIdx.x & 63;
+  const int64_t p = (int64_t)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  if (p >= npairs) return;
+  double th = 0.0;
+  int64_t cr = 0;
+  for (int64_t t = pair_task0[p] + lane; t < pair_task0[p + 1]; t += 64) {
+    th += task_theta[t];
+    cr += task_corr[t];
+  }
+  th = wave_sum(th);
+  cr = wave_sum(cr);
+  if (lane == 0) lk[p] = th * 0.15915494309189533576888376337251436 + (double)cr;
+}
+
+// Converts the uploaded FP64 vertices (already shifted and scaled) into the
+// FP32 hi/lo split and FP32 edge vectors; flags any vertex not exactly
+// representable in FP32.
+__global__ __launch_bounds__(256) void convert_vertices(const double4 *__restrict__ d, int64_t n,
+                                                        float4 *__restrict__ hi,
+                                                        float4 *__restrict__ lo,
+                                                        float4 *__restrict__ ed,
+                                                        int32_t *__restrict__ inexact) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool bad = false;
+  if (v < n) {
+    const double4 p = d[v];
+    const float hx = (float)p.x, hy = (float)p.y, hz = (float)p.z;
+    const float lx = (float)(p.x - (double)hx), ly = (float)(p.y - (double)hy),
+                lz = (float)(p.z - (double)hz);
+    hi[v] = make_float4(hx, hy, hz, 0.f);
+    lo[v] = make_float4(lx, ly, lz, 0.f);
+    bad = (lx != 0.f) || (ly != 0.f) || (lz != 0.f);
+    if (v + 1 < n) {
+      const double4 q = d[v + 1];
+      ed[v] = make_float4((float)(q.x - p.x), (float)(q.y - p.y), (float)(q.z - p.z), 0.f);
+    } else {
+      ed[v] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  }
+  if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(inexact, 1);
+}
+
+}  // namespace
+
+void convert_vertices_launch(hipStream_t s, const double4 *d, int64_t n, float4 *hi, float4 *lo,
+                             float4 *ed, int32_t *inexact) {
+  const int64_t blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(convert_vertices, dim3((unsigned)blocks), dim3(256), 0, s, d, n, hi, lo, ed,
+                     inexact);
+  FLK_HIP(hipGetLastError());
+}
+
+// Rows per lane of the FP32 tile kernel, and rows per wave.
+constexpr int kRowsPerLane = 8;
+constexpr int kRowsPerLane64 = 2;
+
+void ds_run(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, bool fp64,
+            std::vector<double> &lk_out, float *ms_kernel) {
+  Curves &cv = ctx->curves;
+  lk_out.assign(npairs, 0.0);
+  if (ms_kernel) *ms_kernel = 0.f;
+  if (npairs == 0) return;
+  const int R = fp64 ? kRowsPerLane64 : kRowsPerLane;
+  const int rows_per_wave = 64 * R;
+  // Persistent grid: 4 blocks of 4 waves per CU.
+  const int blocks = std::max(1, ctx->num_cus) * 4;
+  const int64_t nwaves = (int64_t)blocks * 4;
+
+  std::vector<DsPair> dp(npairs);
+  int64_t total_pairs = 0, wave_cost = 0;
+  for (int64_t k = 0; k < npairs; ++k) {
+    int a = pairs[2 * k], b = pairs[2 * k + 1];
+    if (cv.n[a] < cv.n[b]) std::swap(a, b);  // rows = longer curve
+    dp[k] = DsPair{cv.off[a], cv.off[b], cv.n[a], cv.n[b]};
+    total_pairs += (int64_t)cv.n[a] * cv.n[b];
+    wave_cost += (int64_t)((cv.n[a] + rows_per_wave - 1) / rows_per_wave) * rows_per_wave * cv.n[b];
+  }
+  // Column chunk: aim for >= 8 tasks per wave over the whole batch, never
+  // below 256 columns.
+  const int64_t target_task = std::max<int64_t>(wave_cost / (nwaves * 8), (int64_t)rows_per_wave * 256);
+  const int64_t chunk_cols = std::max<int64_t>(256, target_task / rows_per_wave);
+
+  std::vector<DsTask> tasks;
+  std::vector<int64_t> prefix, task0(npairs + 1);
+  tasks.reserve(npairs * 4);
+  prefix.reserve(npairs * 4 + 1);
+  int64_t acc = 0;
+  for (int64_t k = 0; k < npairs; ++k) {
+    task0[k] = (int64_t)tasks.size();
+    const DsPair &p = dp[k];
+    if (p.n_r == 0 || p.n_c == 0) continue;
+    for (int32_t r0 = 0; r0 < p.n_r; r0 += rows_per_wave) {
+      for (int64_t c0 = 0; c0 < p.n_c; c0 += chunk_cols) {
+        const int32_t nc = (int32_t)std::min<int64_t>(chunk_cols, p.n_c - c0);
+        tasks.push_back(DsTask{(int32_t)k, r0, (int32_t)c0, nc});
+        prefix.push_back(acc);
+        acc += (int64_t)nc * rows_per_wave;
+      }
+    }
+  }
+  task0[npairs] = (int64_t)tasks.size();
+  prefix.push_back(acc);
+  const int64_t ntasks = (int64_t)tasks.size();
+  ctx->stats.ds_seg_pairs += total_pairs;
+  if (ntasks == 0) return;
+
+  ctx->d_pairs.reserve(sizeof(DsPair) * npairs);
+  ctx->d_tasks.reserve(sizeof(DsTask) * ntasks);
+  ctx->d_cost.reserve(sizeof(int64_t) * (ntasks + 1));
+  ctx->d_task_theta.reserve(sizeof(double) * ntasks);
+  ctx->d_task_corr.reserve(sizeof(int32_t) * ntasks);
+  ctx->d_pair_task0.reserve(sizeof(int64_t) * (npairs + 1));
+  ctx->d_pair_out.reserve(sizeof(double) * npairs);
+  hipStream_t s = ctx->stream;
+  FLK_HIP(hipMemcpyAsync(ctx->d_pairs.ptr, dp.data(), sizeof(DsPair) * npairs, hipMemcpyHostToDevice, s));
+  FLK_HIP(hipMemcpyAsync(ctx->d_tasks.ptr, tasks.data(), sizeof(DsTask) * ntasks, hipMemcpyHostToDevice, s));
+  FLK_HIP(hipMemcpyAsync(ctx->d_cost.ptr, prefix.data(), sizeof(int64_t) * (ntasks + 1), hipMemcpyHostToDevice, s));
+  FLK_HIP(hipMemcpyAsync(ctx->d_pair_task0.ptr, task0.data(), sizeof(int64_t) * (npairs + 1), hipMemcpyHostToDevice, s));
+
+  FLK_HIP(hipEventRecord(ctx->ev0, s));
+  if (fp64) {
+    hipLaunchKernelGGL((ds_tile_f64<kRowsPerLane64>), dim3(blocks), dim3(256), 0, s,
+                       cv.dpts.as<double4>(), ctx->d_pairs.as<DsPair>(), ctx->d_tasks.as<DsTask>(),
+                       ctx->d_cost.as<int64_t>(), ntasks, ctx->d_task_theta.as<double>(),
+                       ctx->d_task_corr.as<int32_t>());
+  } else if (cv.inexact) {
+    hipLaunchKernelGGL((ds_tile_f32<kRowsPerLane, true>), dim3(blocks), dim3(256), 0, s,
+                       cv.hi.as<float4>(), cv.lo.as<float4>(), cv.ed.as<float4>(),
+                       ctx->d_pairs.as<DsPair>(), ctx->d_tasks.as<DsTask>(),
+                       ctx->d_cost.as<int64_t>(), ntasks, ctx->d_task_theta.as<double>(),
+                       ctx->d_task_corr.as<int32_t>());
+  } else {
+    hipLaunchKernelGGL((ds_tile_f32<kRowsPerLane, false>), dim3(blocks), dim3(256), 0, s,
+                       cv.hi.as<float4>(), cv.lo.as<float4>(), cv.ed.as<float4>(),
+                       ctx->d_pairs.as<DsPair>(), ctx->d_tasks.as<DsTask>(),
+                       ctx->d_cost.as<int64_t>(), ntasks, ctx->d_task_theta.as<double>(),
+                       ctx->d_task_corr.as<int32_t>());
+  }
+  FLK_HIP(hipGetLastError());
+  FLK_HIP(hipEventRecord(ctx->ev1, s));
+  const int64_t rblocks = (npairs * 64 + 255) / 256;
+  hipLaunchKernelGGL(ds_pair_reduce, dim3((unsigned)rblocks), dim3(256), 0, s,
+                     ctx->d_task_theta.as<double>(), ctx->d_task_corr.as<int32_t>(),
+                     ctx->d_pair_task0.as<int64_t>(), npairs, ctx->d_pair_out.as<double>());
+  FLK_HIP(hipGetLastError());
+  FLK_HIP(hipMemcpyAsync(lk_out.data(), ctx->d_pair_out.ptr, sizeof(double) * npairs,
+                         hipMemcpyDeviceToHost, s));
+  FLK_HIP(hipStreamSynchronize(s));
+  float ms = 0.f;
+  FLK_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  if (ms_kernel) *ms_kernel = ms;
+  ctx->stats.ds_launches += 1;
+  ctx->stats.ds_waves = (int32_t)nwaves;
+}
+
+}  // namespace flk

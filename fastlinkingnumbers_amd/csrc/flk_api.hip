@@ -1,0 +1,285 @@
+// C ABI of the linking-number engine (include/flk.h).  Translates C++
+// exceptions into status codes and keeps the last error message per context.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "flk_internal.h"
+
+namespace flk {
+void stage_vertices_launch(flk_ctx *ctx);
+}
+
+using flk::Error;
+
+static thread_local std::string g_create_err;
+
+#define FLK_GUARD(ctx, ...)                                  \
+  try {                                                      \
+    __VA_ARGS__                                              \
+  } catch (const flk::Error &e) {                            \
+    if (ctx) (ctx)->err = e.msg;                             \
+    return e.code;                                           \
+  } catch (const std::bad_alloc &) {                         \
+    if (ctx) (ctx)->err = "host allocation failed";          \
+    return FLK_ERR_NOMEM;                                    \
+  } catch (const std::exception &e) {                        \
+    if (ctx) (ctx)->err = e.what();                          \
+    return FLK_ERR_STATE;                                    \
+  }
+
+extern "C" {
+
+int32_t flk_default_params(flk_params *p) {
+  if (!p) return FLK_ERR_ARG;
+  std::memset(p, 0, sizeof *p);
+  p->method = FLK_BARNESHUT;
+  p->criterion = FLK_CRIT_BETA;
+  p->beta_init = 2.0;
+  p->beta_limit = 10.0;
+  p->eps_sq = 1e-24;
+  p->target_error = 0.1;
+  p->err_scale = 0.05;
+  p->fallback_tol = 0.05;
+  p->bh_direct_threshold = 0;
+  p->leaf_size = 1;
+  p->fp64 = 0;
+  p->mlp_log_tol = -7.0;
+  return FLK_OK;
+}
+
+int32_t flk_create(int32_t device, flk_ctx **out) {
+  if (!out) return FLK_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+    g_create_err = "no HIP device available";
+    return FLK_ERR_DEVICE;
+  }
+  if (device < 0 || device >= n) return FLK_ERR_ARG;
+  flk_ctx *ctx = new flk_ctx();
+  ctx->device = device;
+  try {
+    FLK_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    FLK_HIP(hipGetDeviceProperties(&prop, device));
+    ctx->num_cus = prop.multiProcessorCount;
+    ctx->name = prop.gcnArchName;
+    FLK_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    FLK_HIP(hipEventCreate(&ctx->ev0));
+    FLK_HIP(hipEventCreate(&ctx->ev1));
+  } catch (const flk::Error &e) {
+    g_create_err = e.msg;
+    delete ctx;
+    return e.code;
+  }
+  *out = ctx;
+  return FLK_OK;
+}
+
+void flk_destroy(flk_ctx *ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  flk::DevBuf *bufs[] = {&ctx->curves.raw, &ctx->curves.dpts, &ctx->curves.hi, &ctx->curves.lo,
+                         &ctx->curves.ed, &ctx->curves.d_off, &ctx->d_pairs, &ctx->d_tasks,
+                         &ctx->d_cost, &ctx->d_task_theta, &ctx->d_task_corr,
+                         &ctx->d_pair_task0, &ctx->d_pair_out, &ctx->d_pair_corr,
+                         &ctx->d_boxes, &ctx->d_pls, &ctx->d_count, &ctx->d_misc,
+                         &ctx->mlp.d_params, &ctx->mlp.d_norm, &ctx->mlp.d_bf16};
+  for (auto *b : bufs) b->release();
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char *flk_last_error(const flk_ctx *ctx) {
+  return ctx ? ctx->err.c_str() : g_create_err.c_str();
+}
+
+int32_t flk_device_name(const flk_ctx *ctx, char *buf, int32_t len) {
+  if (!ctx || !buf || len <= 0) return FLK_ERR_ARG;
+  std::snprintf(buf, (size_t)len, "%s", ctx->name.c_str());
+  return FLK_OK;
+}
+
+int32_t flk_set_curves(flk_ctx *ctx, const double *xyz, const int64_t *offsets, int32_t ncurves) {
+  if (!ctx || ncurves < 0 || (ncurves > 0 && !offsets)) return FLK_ERR_ARG;
+  FLK_GUARD(ctx, {
+    FLK_HIP(hipSetDevice(ctx->device));
+    flk::Curves &cv = ctx->curves;
+    if (offsets && offsets[0] != 0) throw Error(FLK_ERR_ARG, "offsets[0] must be 0");
+    for (int32_t c = 0; c < ncurves; ++c)
+      if (offsets[c + 1] < offsets[c]) throw Error(FLK_ERR_ARG, "offsets must be non-decreasing");
+    for (int32_t c = 0; c < ncurves; ++c)
+      if (offsets[c + 1] - offsets[c] > INT32_MAX - 1)
+        throw Error(FLK_ERR_ARG, "a curve has more than 2^31-2 vertices");
+    const int64_t nv = ncurves ? offsets[ncurves] : 0;
+    if (nv > 0 && !xyz) throw Error(FLK_ERR_ARG, "xyz is NULL");
+    cv.ncurves = ncurves;
+    cv.in_off.assign(offsets, offsets + ncurves + 1);
+    if (ncurves == 0) cv.in_off.assign(1, 0);
+    cv.n.resize(ncurves);
+    cv.off.resize(ncurves + 1);
+    for (int32_t c = 0; c < ncurves; ++c) {
+      cv.n[c] = (int32_t)(offsets[c + 1] - offsets[c]);
+      cv.off[c] = offsets[c] + c;  // one closing vertex per curve
+    }
+    cv.off[ncurves] = nv + ncurves;
+    cv.nvert_dev = nv + ncurves;
+    hipStream_t s = ctx->stream;
+    cv.raw.reserve(sizeof(double) * 3 * std::max<int64_t>(nv, 1));
+    if (nv) FLK_HIP(hipMemcpyAsync(cv.raw.ptr, xyz, sizeof(double) * 3 * nv, hipMemcpyHostToDevice, s));
+    flk::curve_boxes_run(ctx, cv.box);
+    // Origin and power-of-two scale: stored = (x - origin) * scale with the
+    // model extent near 1.  The origin stays 0 unless the model is far from it
+    // (then it is snapped to a coarse grid so fp32 inputs remain exact).
+    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (int32_t c = 0; c < ncurves; ++c)
+      if (cv.box[6 * c] <= cv.box[6 * c + 3])
+        for (int d = 0; d < 3; ++d) {
+          lo[d] = std::min(lo[d], cv.box[6 * c + d]);
+          hi[d] = std::max(hi[d], cv.box[6 * c + 3 + d]);
+        }
+    double extent = 0.0, far = 0.0;
+    for (int d = 0; d < 3; ++d)
+      if (lo[d] <= hi[d]) {
+        extent = std::max(extent, hi[d] - lo[d]);
+        far = std::max(far, std::fabs(0.5 * (lo[d] + hi[d])));
+      }
+    cv.origin[0] = cv.origin[1] = cv.origin[2] = 0.0;
+    if (extent > 0 && far > 4 * extent) {
+      const double grid = std::ldexp(1.0, (int)std::floor(std::log2(extent)));
+      for (int d = 0; d < 3; ++d) cv.origin[d] = grid * std::round(0.5 * (lo[d] + hi[d]) / grid);
+    }
+    double span = 0.0;
+    for (int d = 0; d < 3; ++d)
+      if (lo[d] <= hi[d])
+        span = std::max(span, std::max(std::fabs(hi[d] - cv.origin[d]), std::fabs(lo[d] - cv.origin[d])));
+    cv.scale = span > 0 ? std::ldexp(1.0, -(int)std::ceil(std::log2(span))) : 1.0;
+    if (!std::isfinite(cv.scale) || cv.scale == 0) cv.scale = 1.0;
+    cv.d_off.reserve(sizeof(int64_t) * (ncurves + 1));
+    cv.dpts.reserve(sizeof(double4) * std::max<int64_t>(cv.nvert_dev, 1));
+    cv.hi.reserve(sizeof(float4) * std::max<int64_t>(cv.nvert_dev, 1));
+    cv.lo.reserve(sizeof(float4) * std::max<int64_t>(cv.nvert_dev, 1));
+    cv.ed.reserve(sizeof(float4) * std::max<int64_t>(cv.nvert_dev, 1));
+    cv.inexact = false;
+    if (cv.nvert_dev > 0) {
+      flk::stage_vertices_launch(ctx);
+      ctx->d_count.reserve(sizeof(int32_t));
+      FLK_HIP(hipMemsetAsync(ctx->d_count.ptr, 0, sizeof(int32_t), s));
+      flk::convert_vertices_launch(s, cv.dpts.as<double4>(), cv.nvert_dev, cv.hi.as<float4>(),
+                                   cv.lo.as<float4>(), cv.ed.as<float4>(),
+                                   ctx->d_count.as<int32_t>());
+      int32_t flag = 0;
+      FLK_HIP(hipMemcpyAsync(&flag, ctx->d_count.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      FLK_HIP(hipStreamSynchronize(s));
+      cv.inexact = flag != 0;
+    }
+    return FLK_OK;
+  })
+}
+
+int32_t flk_curve_boxes(flk_ctx *ctx, double *boxes_out) {
+  if (!ctx || (!boxes_out && ctx->curves.ncurves)) return FLK_ERR_ARG;
+  FLK_GUARD(ctx, {
+    std::copy(ctx->curves.box.begin(), ctx->curves.box.end(), boxes_out);
+    return FLK_OK;
+  })
+}
+
+int32_t flk_potential_links(flk_ctx *ctx, const double *boxes, int32_t ncurves,
+                            const int32_t **pairs, int64_t *npairs) {
+  if (!ctx || !pairs || !npairs || ncurves < 0) return FLK_ERR_ARG;
+  FLK_GUARD(ctx, {
+    FLK_HIP(hipSetDevice(ctx->device));
+    const double *b = boxes;
+    if (!b) {
+      if (ncurves != ctx->curves.ncurves)
+        throw Error(FLK_ERR_ARG, "ncurves does not match the uploaded curves");
+      b = ctx->curves.box.data();
+    }
+    flk::pls_run(ctx, b, ncurves, ctx->pls_pairs);
+    *pairs = ctx->pls_pairs.data();
+    *npairs = (int64_t)ctx->pls_pairs.size() / 2;
+    return FLK_OK;
+  })
+}
+
+int32_t flk_linking_numbers(flk_ctx *ctx, const int32_t *pairs, int64_t npairs,
+                            const flk_params *params, double *lk, double *err_est,
+                            uint32_t *flags) {
+  if (!ctx || npairs < 0 || (npairs > 0 && (!pairs || !lk))) return FLK_ERR_ARG;
+  FLK_GUARD(ctx, {
+    FLK_HIP(hipSetDevice(ctx->device));
+    flk_params prm;
+    if (params) prm = *params;
+    else flk_default_params(&prm);
+    const auto t0 = std::chrono::steady_clock::now();
+    std::memset(&ctx->stats, 0, sizeof ctx->stats);
+    const flk::Curves &cv = ctx->curves;
+    for (int64_t k = 0; k < 2 * npairs; ++k)
+      if (pairs[k] < 0 || pairs[k] >= cv.ncurves) throw Error(FLK_ERR_ARG, "pair index out of range");
+    for (int64_t k = 0; k < npairs; ++k)
+      ctx->stats.seg_pairs += (int64_t)cv.n[pairs[2 * k]] * cv.n[pairs[2 * k + 1]];
+    std::vector<double> v, e(npairs, 0.0);
+    std::vector<uint32_t> f(npairs, 0u);
+    if (prm.method == FLK_DIRECTSUM) {
+      float ms = 0.f;
+      flk::ds_run(ctx, pairs, npairs, prm.fp64 != 0, v, &ms);
+      ctx->stats.ms_kernel = ms;
+      ctx->stats.ms_ds_kernel = ms;
+    } else if (prm.method == FLK_BARNESHUT) {
+      flk::bh_run(ctx, pairs, npairs, prm, v, e, f);
+    } else {
+      throw Error(FLK_ERR_ARG, "invalid method");
+    }
+    // Fallback and judge: FP64 direct sum for every pair that lands farther
+    // than fallback_tol from an integer (or is not finite).
+    if (prm.fallback_tol > 0 && !(prm.method == FLK_DIRECTSUM && prm.fp64)) {
+      std::vector<int32_t> redo;
+      std::vector<int64_t> idx;
+      for (int64_t k = 0; k < npairs; ++k) {
+        const double d = std::fabs(v[k] - std::round(v[k]));
+        if (!(d <= prm.fallback_tol)) {
+          redo.push_back(pairs[2 * k]);
+          redo.push_back(pairs[2 * k + 1]);
+          idx.push_back(k);
+        }
+      }
+      if (!idx.empty()) {
+        std::vector<double> w;
+        float ms = 0.f;
+        flk::ds_run(ctx, redo.data(), (int64_t)idx.size(), true, w, &ms);
+        ctx->stats.ms_fallback = ms;
+        ctx->stats.fallbacks = (int64_t)idx.size();
+        for (size_t t = 0; t < idx.size(); ++t) {
+          v[idx[t]] = w[t];
+          f[idx[t]] |= FLK_PAIR_FALLBACK;
+        }
+      }
+    }
+    for (int64_t k = 0; k < npairs; ++k) {
+      if (std::isnan(v[k])) f[k] |= FLK_PAIR_NAN;
+      lk[k] = v[k];
+      if (err_est) err_est[k] = e[k];
+      if (flags) flags[k] = f[k];
+    }
+    ctx->stats.ms_total =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return FLK_OK;
+  })
+}
+
+int32_t flk_get_stats(const flk_ctx *ctx, flk_stats *out) {
+  if (!ctx || !out) return FLK_ERR_ARG;
+  *out = ctx->stats;
+  return FLK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,148 @@
+// Internal declarations of the MI355X linking-number engine (libflk.so).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "flk.h"
+
+#define FLK_HIP(call)                                                          \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      throw flk::Error(FLK_ERR_HIP, std::string(#call) + ": " +                \
+                                        hipGetErrorString(e_));                \
+  } while (0)
+
+namespace flk {
+
+struct Error {
+  int32_t code;
+  std::string msg;
+  Error(int32_t c, std::string m) : code(c), msg(std::move(m)) {}
+};
+
+// Device buffer owned by a context; grows, never shrinks.
+struct DevBuf {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  void reserve(size_t n) {
+    if (n <= bytes) return;
+    if (ptr) FLK_HIP(hipFree(ptr));
+    ptr = nullptr;
+    bytes = 0;
+    if (n == 0) return;
+    FLK_HIP(hipMalloc(&ptr, n));
+    bytes = n;
+  }
+  template <typename T> T *as() const { return static_cast<T *>(ptr); }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+};
+
+// One segment-pair work description for the direct-sum tile kernel: rows
+// [row0, row0 + 64*R) of the row curve against columns [col0, col0 + ncols)
+// of the column curve.
+struct DsPair {
+  int64_t off_r, off_c;  // first vertex of each curve in the vertex arrays
+  int32_t n_r, n_c;      // vertex (= segment) counts
+};
+struct DsTask {
+  int32_t pair;
+  int32_t row0;
+  int32_t col0;
+  int32_t ncols;
+};
+
+// Barnes–Hut node record (FP32, 160 bytes): centre, radius, monopole c, dipole
+// D (row-major D[k][j]), quadrupole Q[k] symmetric in (i,j) stored as
+// xx xy xz yy yz zz per k, norms, children and segment range.
+struct BhNode {
+  float ctr[3];
+  float radius;
+  float c[3];
+  float cn;
+  float D[9];
+  float dn;
+  float Q[18];
+  float qn;
+  int32_t child[2];     // >= 0 internal node index; < 0: leaf bucket -(k+1)
+  int32_t seg0, nseg;   // contiguous range in the tree's segment order
+  float rsq;            // radius squared as the reference computes it
+  float pad[2];
+};
+
+struct Curves {
+  int32_t ncurves = 0;
+  std::vector<int64_t> off;      // host offsets into the device vertex arrays (n+1 per curve)
+  std::vector<int32_t> n;        // vertices per curve
+  std::vector<double> box;       // ncurves x 6, FP64, in input coordinates
+  std::vector<uint8_t> exact;    // curve coordinates exactly representable in FP32
+  double origin[3] = {0, 0, 0};  // input = origin + scale^-1 * stored
+  double scale = 1.0;            // power of two
+  std::vector<double> xyz;       // host copy of the input (FP64 AoS)
+  std::vector<int64_t> in_off;   // input offsets
+  DevBuf raw;                    // input FP64 AoS (no closing vertex)
+  DevBuf dpts;                   // double4 per vertex, shifted+scaled, +1 closing vertex per curve
+  DevBuf hi, lo, ed;             // float4 per vertex (+1 closing vertex per curve)
+  DevBuf d_off;                  // int64 device offsets (ncurves + 1)
+  int64_t nvert_dev = 0;
+  bool inexact = false;          // some stored coordinate needs the FP32 lo part
+};
+
+struct MlpState {
+  std::vector<float> params;  // FLK_MLP_NPARAM FP32 masters
+  std::vector<float> norm;    // mean[12], std[12]
+  bool loaded = false;
+  DevBuf d_params, d_norm, d_bf16;
+};
+
+}  // namespace flk
+
+struct flk_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int num_cus = 0;
+  std::string name;
+  std::string err;
+  flk::Curves curves;
+  flk_stats stats{};
+  std::vector<int32_t> pls_pairs;
+  flk::MlpState mlp;
+  // scratch
+  flk::DevBuf d_pairs, d_tasks, d_cost, d_task_theta, d_task_corr, d_pair_task0,
+      d_pair_out, d_pair_corr, d_boxes, d_pls, d_count, d_misc;
+};
+
+namespace flk {
+
+// Direct sum over a set of curve pairs.  theta_out[k] receives the sum of the
+// per-segment-pair half solid angles (radians) and corr_out[k] the integer
+// branch corrections; the linking number is theta/(2 pi) + corr.
+void ds_run(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, bool fp64,
+            std::vector<double> &lk_out, float *ms_kernel);
+
+void bh_run(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, const flk_params &prm,
+            std::vector<double> &lk, std::vector<double> &err, std::vector<uint32_t> &flags);
+
+void pls_run(flk_ctx *ctx, const double *boxes, int32_t ncurves, std::vector<int32_t> &out);
+
+void curve_boxes_run(flk_ctx *ctx, std::vector<double> &boxes);
+void convert_vertices_launch(hipStream_t s, const double4 *d, int64_t n, float4 *hi, float4 *lo,
+                             float4 *ed, int32_t *inexact);
+
+int64_t mlp_samples(flk_ctx *ctx, int32_t a, int32_t b, int64_t cap, uint64_t seed, float *x,
+                    float *y);
+void mlp_forward(flk_ctx *ctx, const float *x, int64_t n, float *y);
+float mlp_gradient(flk_ctx *ctx, const float *x, const float *y, int64_t n, float *grad);
+float mlp_train(flk_ctx *ctx, const float *x, const float *y, int64_t n, int epochs, int batch,
+                float lr, uint64_t seed);
+
+}  // namespace flk

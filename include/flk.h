@@ -1,0 +1,161 @@
+/*
+ * flk.h — C ABI of the MI355X linking-number engine (libflk.so).
+ *
+ * Plain C types only: host pointers, sizes, status codes.  The engine owns all
+ * device memory; callers never see HIP or torch types.  One context = one HIP
+ * stream on one device; a context is not thread-safe, separate contexts are
+ * independent.  Every call is synchronous with respect to the caller.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the reference repository antequ/fastlinkingnumbers).
+ */
+#ifndef FLK_H_
+#define FLK_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes: 0 = success, negative = error (message via flk_last_error). */
+#define FLK_OK 0
+#define FLK_ERR_ARG (-1)
+#define FLK_ERR_HIP (-2)
+#define FLK_ERR_NOMEM (-3)
+#define FLK_ERR_STATE (-4)
+#define FLK_ERR_DEVICE (-5)
+
+/* Linking-number methods: the --method switch, src/main.cpp:34-42. */
+#define FLK_DIRECTSUM 0
+#define FLK_BARNESHUT 1
+
+/* Barnes–Hut opening criteria. */
+#define FLK_CRIT_BETA 0 /* |r| > beta (r1 + r2), src/compute_link.cpp:573-574 */
+#define FLK_CRIT_MLP 1  /* learned far-field error predictor (no reference counterpart) */
+
+/* Per-pair result flags (bit set in flags[k]). */
+#define FLK_PAIR_FALLBACK 1u /* recomputed by the FP64 direct sum */
+#define FLK_PAIR_NAN 2u      /* the final sum is NaN */
+#define FLK_PAIR_RETRY 4u    /* Barnes–Hut re-run with a larger beta */
+#define FLK_PAIR_DIRECT 8u   /* Barnes–Hut request served by direct summation */
+
+typedef struct flk_ctx flk_ctx;
+
+typedef struct flk_params {
+  int32_t method;       /* FLK_DIRECTSUM or FLK_BARNESHUT */
+  int32_t criterion;    /* FLK_CRIT_BETA or FLK_CRIT_MLP */
+  double beta_init;     /* 2.0,  src/define_args.h:51 */
+  double beta_limit;    /* 10.0, src/define_args.h:54 */
+  double eps_sq;        /* 1e-24, src/linking_number_certificate.cpp:24 */
+  double target_error;  /* 0.1,   src/linking_number_certificate.cpp:25 */
+  double err_scale;     /* 0.05,  src/linking_number_certificate.cpp:26 */
+  double fallback_tol;  /* a pair whose sum is farther than this from an integer
+                           is recomputed by the FP64 direct sum (<= 0 disables) */
+  int64_t bh_direct_threshold; /* BH pairs with n1*n2 below this use the direct sum */
+  int32_t leaf_size;    /* segments per Barnes–Hut leaf bucket (1 = reference) */
+  int32_t fp64;         /* 1: direct sum in FP64 instead of FP32 */
+  double mlp_log_tol;   /* MLP criterion: accept far field when the predicted
+                           log10 |error| is below this */
+  int32_t reserved[8];
+} flk_params;
+
+typedef struct flk_stats {
+  double ms_total;          /* wall time of the last flk_linking_numbers call */
+  double ms_kernel;         /* device time of its main kernel(s), HIP events */
+  double ms_ds_kernel;      /* device time of the direct-sum tile kernel */
+  double ms_bh_traverse;    /* device time of the Barnes–Hut traversal */
+  double ms_bh_build;       /* device time of tree + moment construction */
+  double ms_fallback;       /* device time of FP64 fallback recomputation */
+  int64_t seg_pairs;        /* sum of n1*n2 over the requested pairs */
+  int64_t ds_seg_pairs;     /* segment pairs evaluated by direct summation */
+  int64_t bh_visits;        /* Barnes–Hut node-pair visits */
+  int64_t bh_far;           /* far-field evaluations */
+  int64_t bh_leaf_pairs;    /* exact segment pairs evaluated in leaf buckets */
+  int64_t fallbacks;        /* pairs recomputed by the FP64 direct sum */
+  int64_t retries;          /* pairs re-run with a larger beta */
+  int32_t ds_launches;      /* direct-sum tile kernel launches */
+  int32_t ds_waves;         /* waves per direct-sum launch */
+  int64_t reserved[8];
+} flk_stats;
+
+/* Fills *p with the reference defaults (method Barnes–Hut, beta 2/10, ...). */
+int32_t flk_default_params(flk_params *p);
+
+/* Context lifecycle.  No reference counterpart (the reference is CPU-only);
+ * replaces the implicit OpenMP runtime of src/linking_number_certificate.cpp. */
+int32_t flk_create(int32_t device, flk_ctx **out);
+void flk_destroy(flk_ctx *ctx);
+const char *flk_last_error(const flk_ctx *ctx);
+int32_t flk_device_name(const flk_ctx *ctx, char *buf, int32_t len);
+
+/* Uploads closed polylines: xyz is AoS FP64 (3 * offsets[ncurves] doubles),
+ * curve c owns vertices [offsets[c], offsets[c+1]).  Replaces the per-pair
+ * Curve::GetStartPoints / GetLoopedStartPoints copies made by
+ * src/compute_link.cpp:204-205 and :131-132 (the vertex list of a discretized
+ * curve is its start-point list, include/curve.h:105-108). */
+int32_t flk_set_curves(flk_ctx *ctx, const double *xyz, const int64_t *offsets,
+                       int32_t ncurves);
+
+/* Curve bounding boxes of the uploaded polylines, ncurves x (min xyz, max xyz)
+ * in FP64.  Replaces PrecomputeCurveBoundingBoxes,
+ * src/potential_link_search.cpp:68-73 (polyline case of curve.cpp:331-343). */
+int32_t flk_curve_boxes(flk_ctx *ctx, double *boxes_out);
+
+/* Potential-link search over curve boxes (ncurves x 6 FP64; NULL = use the
+ * boxes of the uploaded curves).  Writes the sorted unique pair list (i < j,
+ * closed-interval overlap) into a context-owned array valid until the next
+ * call.  Replaces GetPotentialLinksUniqueList,
+ * src/potential_link_search.cpp:88-127. */
+int32_t flk_potential_links(flk_ctx *ctx, const double *boxes, int32_t ncurves,
+                            const int32_t **pairs, int64_t *npairs);
+
+/* Linking numbers of curve pairs (2 * npairs indices into the uploaded curves).
+ * lk[k] receives the summed Gauss integral of pair k, err_est[k] (may be NULL)
+ * the Barnes–Hut error estimate (0 for direct summation), flags[k] (may be
+ * NULL) FLK_PAIR_* bits.  Replaces the per-pair loops of
+ * ComputeLinkingNumbersDirectSum (src/linking_number_certificate.cpp:241-338,
+ * calling src/compute_link.cpp:126-280) and ComputeLinkingNumbersBarnesHut
+ * (src/linking_number_certificate.cpp:102-229, calling
+ * src/compute_link.cpp:294-934). */
+int32_t flk_linking_numbers(flk_ctx *ctx, const int32_t *pairs, int64_t npairs,
+                            const flk_params *params, double *lk, double *err_est,
+                            uint32_t *flags);
+
+/* Statistics of the last flk_linking_numbers call. */
+int32_t flk_get_stats(const flk_ctx *ctx, flk_stats *out);
+
+/* Learned opening criterion (no reference counterpart; it replaces the beta
+ * test of src/compute_link.cpp:573-574 and :608).  Weights are FP32 masters of
+ * the 12-64-64-1 SiLU MLP: W1[64x12] b1[64] W2[64x64] b2[64] W3[64] b3[1]
+ * (5,057 floats), plus 24 floats of feature normalisation (mean[12], std[12]).
+ */
+#define FLK_MLP_NPARAM 5057
+#define FLK_MLP_NFEAT 12
+int32_t flk_mlp_set_weights(flk_ctx *ctx, const float *params, const float *norm24);
+int32_t flk_mlp_get_weights(const flk_ctx *ctx, float *params, float *norm24);
+/* Forward pass of the bf16-MFMA inference kernel: x is n x 12 FP32 features
+ * (raw, normalised inside), y receives n predicted log10 errors. */
+int32_t flk_mlp_forward(flk_ctx *ctx, const float *x, int64_t n, float *y);
+/* One full-batch gradient of the mean squared error over (x, y) with the
+ * bf16-MFMA forward/backward kernels; grad receives FLK_MLP_NPARAM floats,
+ * loss the mean loss. */
+int32_t flk_mlp_gradient(flk_ctx *ctx, const float *x, const float *y, int64_t n,
+                         float *grad, float *loss);
+/* Adam training on the device: epochs over (x, y) with minibatch size batch,
+ * learning rate lr, shuffling seed.  Returns the final training loss. */
+int32_t flk_mlp_train(flk_ctx *ctx, const float *x, const float *y, int64_t n,
+                      int32_t epochs, int32_t batch, float lr, uint64_t seed,
+                      float *final_loss);
+/* Node-pair training samples from the Barnes–Hut trees of curve pair (a, b):
+ * up to cap samples of 12 features and the label log10 |exact - expansion|,
+ * where exact is the direct sum over the two nodes' segments and expansion
+ * the second-order far field.  Returns the number written in *count. */
+int32_t flk_mlp_samples(flk_ctx *ctx, int32_t a, int32_t b, int64_t cap,
+                        uint64_t seed, float *x, float *y, int64_t *count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLK_H_ */
