@@ -1302,6 +1302,27 @@ double orc_ds_fast(const double *p, int64_t n1, const double *q, int64_t n2) {
   auto P = make_pts(p, n1), Q = make_pts(q, n2);
   return ds_fast(P, Q);
 }
+int orc_ds_pairs(const double *xyz, const int64_t *offsets, int32_t ncurves,
+                 const int32_t *pairs, int64_t npairs, double *out) {
+  try {
+    std::vector<std::vector<V3>> pts(ncurves);
+    std::vector<char> used(ncurves, 0);
+    for (int64_t k = 0; k < 2 * npairs; ++k) used.at(pairs[k]) = 1;
+    for (int32_t c = 0; c < ncurves; ++c)
+      if (used[c]) pts[c] = make_pts(xyz + 3 * offsets[c], offsets[c + 1] - offsets[c]);
+    const bool few = ncurves < kParallelCurves;
+#pragma omp parallel for schedule(dynamic) if (!few)
+    for (int64_t k = 0; k < npairs; ++k) {
+      const auto &p = pts[pairs[2 * k]], &q = pts[pairs[2 * k + 1]];
+      out[k] = few ? ds_rows(p, q, 0, (int64_t)p.size(), true) : ds_fast(p, q);
+    }
+    return 0;
+  } catch (const std::exception &e) {
+    orc::g_err = e.what();
+    return -1;
+  }
+}
+
 double orc_segment_pair_term(const double *p0, const double *p1, const double *q0,
                              const double *q1) {
   V3 P0(p0[0], p0[1], p0[2]), P1(p1[0], p1[1], p1[2]);

@@ -71,6 +71,14 @@ double orc_segment_pair_term(const double *p0, const double *p1, const double *q
 double orc_bh_leaf_term(const double *m1, const double *t1, const double *m2,
                         const double *t2);
 
+/* The pair loop of ComputeLinkingNumbersDirectSum
+ * (src/linking_number_certificate.cpp:263-293): with ncurves >= 6 OpenMP runs
+ * over pairs and each pair uses the running-product sum (DirectFast), else each
+ * pair uses the one-arctan sum parallel over rows.  Curves are xyz AoS with
+ * offsets[ncurves+1]; out receives one sum per pair. */
+int orc_ds_pairs(const double *xyz, const int64_t *offsets, int32_t ncurves,
+                 const int32_t *pairs, int64_t npairs, double *out);
+
 /* ---- Barnes–Hut (src/compute_link.cpp:294-934) ---- */
 /* One closed polyline pair.  beta is the opening parameter (itolsq = beta^2),
  * parallel selects the BFS(<75k)+DFS path, serial the float-truncated DFS.

@@ -55,6 +55,7 @@ def lib():
             "orc_ds_fast": (C.c_double, [_D, C.c_int64, _D, C.c_int64]),
             "orc_ds_rows": (C.c_double, [_D, C.c_int64, _D, C.c_int64, C.c_int64, C.c_int64, C.c_int]),
             "orc_segment_pair_term": (C.c_double, [_D, _D, _D, _D]),
+            "orc_ds_pairs": (C.c_int, [_D, _P(C.c_int64), C.c_int32, _P(C.c_int32), C.c_int64, _D]),
             "orc_bh_leaf_term": (C.c_double, [_D, _D, _D, _D]),
             "orc_bh_pair": (C.c_double, [_D, C.c_int64, _D, C.c_int64, C.c_double, C.c_int, _D,
                                          _P(C.c_int64)]),
@@ -193,6 +194,19 @@ def ds_fast(p, q):
 def ds_rows(p, q, r0, r1, parallel=True):
     p, q = _pts(p), _pts(q)
     return lib().orc_ds_rows(_dp(p), len(p), _dp(q), len(q), r0, r1, int(parallel))
+
+
+def ds_pairs(xyz, offsets, pairs):
+    """The reference's direct-sum pair loop over a packed curve set."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1)
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1)
+    out = np.zeros(len(pairs) // 2)
+    rc = lib().orc_ds_pairs(_dp(xyz), offsets.ctypes.data_as(_P(C.c_int64)), len(offsets) - 1,
+                            pairs.ctypes.data_as(_P(C.c_int32)), len(pairs) // 2, _dp(out))
+    if rc != 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    return out
 
 
 def segment_pair_term(p0, p1, q0, q1):
