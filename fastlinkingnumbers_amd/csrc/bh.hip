@@ -1,9 +1,817 @@
-// Two-tree Barnes–Hut on the device (placeholder until the traversal lands).
+// Two-tree Barnes–Hut on the device (gfx950).
+//
+// Reference: src/compute_link.cpp:294-934 (trees, moments, far field, node-pair
+// traversal) and src/linking_number_certificate.cpp:102-229 (adaptive beta).
+//
+// Trees.  All segments of the curves in Barnes–Hut pairs are sorted once by
+// (curve, 48-bit Morton code of the segment midpoint in the curve's box), a
+// single radix sort.  Each curve's tree is the balanced median split of its
+// Morton-ordered segments (left half floor(n/2)), down to leaf buckets of at
+// most leaf_size segments; a subtree is a contiguous range of the sorted order
+// (the reference's ModifiedKdBVH.h:128-131 property).  Nodes are created level
+// by level with scan-assigned child slots (deterministic numbering).
+//
+// Moments (reference compute_link.cpp:333-389) are built bottom-up in FP64
+// about each node's box centre: leaf buckets sum their segments' closed-form
+// moments, internal nodes shift their children (Eqs. 21-23), then the records
+// are stored in FP32.  The centre keeps an FP32 hi/lo split so node
+// separations are exact to FP32 precision of the separation itself.
+//
+// Traversal.  Work items (pair, node1, node2) live in level buffers; one
+// kernel per level classifies every item: leaf-bucket pairs go to a leaf list
+// evaluated exactly by a second kernel, pairs with |r|^2 > beta^2 (r1+r2)^2
+// are evaluated with the second-order far field (reference
+// compute_link.cpp:410-524, Supplemental Eqs. 29-30) and the Eq. 14 error
+// estimate, the rest split the larger node (a leaf is never split) into the
+// next level.  Levels are processed in chunks with a LIFO stack, so memory is
+// bounded by depth x chunk whatever the frontier width.  Per-pair sums and
+// error estimates are FP64, reduced per wave before one atomic add.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
 #include "flk_internal.h"
 
 namespace flk {
-void bh_run(flk_ctx *, const int32_t *, int64_t, const flk_params &, std::vector<double> &,
-            std::vector<double> &, std::vector<uint32_t> &) {
-  throw Error(FLK_ERR_STATE, "Barnes-Hut is not available in this build");
+
+// ---------------------------------------------------------------- records
+struct BhGeo {      // 64 bytes: what the classification reads
+  float cx, cy, cz, radius;   // centre (hi part) and half box diagonal
+  float lx, ly, lz, rsq;      // centre lo part, radius^2 as 0.25 |diag|^2
+  int32_t c0, c1;             // children (c0 < 0: leaf bucket)
+  int32_t seg0, nseg;         // range in the sorted segment order
+  float cn, dn, qn, pad;      // moment norms
+};
+struct BhMom {      // 128 bytes: read by far-field evaluations
+  float c[3];
+  float D[9];       // D[k*3+j] = int r'_k (r - ctr)_j
+  float Q[18];      // per k: xx xy xz yy yz zz of int r'_k (r-ctr)_i (r-ctr)_j
+  float pad[2];
+};
+struct BhItem {
+  int32_t pair, a, b, pad;
+};
+struct BhRange {    // node range during construction
+  int32_t lo, hi;
+};
+struct Mom64 {      // FP64 moments during construction
+  double box[6];
+  double c[3];
+  double D[9];
+  double Q[18];
+};
+
+namespace {
+
+constexpr float kInv2PiF = 0.15915494309189533577f;
+constexpr float kInv4PiF = 0.07957747154594766788f;
+
+__device__ __forceinline__ uint64_t spread16(uint64_t v) {
+  v &= 0xffff;
+  v = (v | (v << 16)) & 0x0000ff0000ffULL;
+  v = (v | (v << 8)) & 0x00f00f00f00fULL;
+  v = (v | (v << 4)) & 0x0c30c30c30c3ULL;
+  v = (v | (v << 2)) & 0x249249249249ULL;
+  return v;
 }
+
+// One thread per segment of the Barnes–Hut curves: key = curve rank << 48 |
+// Morton code of the midpoint in the curve's box.
+__global__ __launch_bounds__(256) void bh_keys(const double4 *__restrict__ pts,
+                                               const int64_t *__restrict__ seg_off,
+                                               const int64_t *__restrict__ vtx_off,
+                                               const double *__restrict__ cbox, int32_t ncurves,
+                                               int64_t nseg, uint64_t *__restrict__ keys,
+                                               uint32_t *__restrict__ vals) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nseg) return;
+  int lo = 0, hi = ncurves;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (seg_off[mid] <= g) lo = mid;
+    else hi = mid;
+  }
+  const int c = lo;
+  const int64_t v = vtx_off[c] + (g - seg_off[c]);
+  const double4 p = pts[v], q = pts[v + 1];
+  const double m[3] = {0.5 * (p.x + q.x), 0.5 * (p.y + q.y), 0.5 * (p.z + q.z)};
+  uint64_t code = 0;
+  for (int d = 0; d < 3; ++d) {
+    const double ext = cbox[6 * c + 3 + d] - cbox[6 * c + d];
+    double u = ext > 0 ? (m[d] - cbox[6 * c + d]) / ext : 0.0;
+    u = fmin(fmax(u, 0.0), 1.0);
+    const uint64_t qd = (uint64_t)fmin(u * 65536.0, 65535.0);
+    code |= spread16(qd) << (2 - d);
+  }
+  keys[g] = ((uint64_t)c << 48) | code;
+  vals[g] = (uint32_t)v;
+}
+
+// Splits every node of one level with more than `leaf` segments: children get
+// slots 2*scan[k], 2*scan[k]+1 of the next level.
+__global__ __launch_bounds__(256) void bh_split_flags(const BhRange *__restrict__ lev, int32_t n,
+                                                      int32_t leaf, int32_t *__restrict__ flag) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  flag[k] = (lev[k].hi - lev[k].lo) > leaf ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void bh_split(const BhRange *__restrict__ lev, int32_t n,
+                                                int32_t base, int32_t next_base,
+                                                const int32_t *__restrict__ flag,
+                                                const int32_t *__restrict__ scan,
+                                                BhRange *__restrict__ next,
+                                                int2 *__restrict__ child) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const BhRange r = lev[k];
+  if (flag[k]) {
+    const int s = 2 * scan[k];
+    const int mid = r.lo + (r.hi - r.lo) / 2;
+    next[s] = BhRange{r.lo, mid};
+    next[s + 1] = BhRange{mid, r.hi};
+    child[base + k] = make_int2(next_base + s, next_base + s + 1);
+  } else {
+    child[base + k] = make_int2(-1, -1);
+  }
+}
+
+__device__ __forceinline__ int qidx(int i, int j) {  // symmetric 3x3 -> 6
+  return i <= j ? (i == 0 ? j : (i == 1 ? 2 + j : 5)) : (j == 0 ? i : (j == 1 ? 2 + i : 5));
+}
+
+// Bottom-up moments of one level (children already done).  Leaves sum the
+// closed-form moments of their segments about the leaf box centre:
+// c += t, D += t (m - ctr)^T, Q_k += t_k (t t^T / 12 + (m-ctr)(m-ctr)^T).
+__global__ __launch_bounds__(128) void bh_moments(const BhRange *__restrict__ lev, int32_t n,
+                                                  int32_t base, const int2 *__restrict__ child,
+                                                  const uint32_t *__restrict__ order,
+                                                  const double4 *__restrict__ pts,
+                                                  Mom64 *__restrict__ m64) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int id = base + k;
+  const int2 ch = child[id];
+  Mom64 M;
+  for (int t = 0; t < 3; ++t) {
+    M.box[t] = DBL_MAX;
+    M.box[3 + t] = -DBL_MAX;
+    M.c[t] = 0;
+  }
+  for (int t = 0; t < 9; ++t) M.D[t] = 0;
+  for (int t = 0; t < 18; ++t) M.Q[t] = 0;
+  if (ch.x < 0) {
+    const BhRange r = lev[k];
+    for (int s = r.lo; s < r.hi; ++s) {
+      const double4 a = pts[order[s]], b = pts[order[s] + 1];
+      const double pa[3] = {a.x, a.y, a.z}, pb[3] = {b.x, b.y, b.z};
+      for (int d = 0; d < 3; ++d) {
+        M.box[d] = fmin(M.box[d], fmin(pa[d], pb[d]));
+        M.box[3 + d] = fmax(M.box[3 + d], fmax(pa[d], pb[d]));
+      }
+    }
+    double ctr[3];
+    for (int d = 0; d < 3; ++d) ctr[d] = 0.5 * (M.box[3 + d] + M.box[d]);
+    for (int s = r.lo; s < r.hi; ++s) {
+      const double4 a = pts[order[s]], b = pts[order[s] + 1];
+      const double t[3] = {b.x - a.x, b.y - a.y, b.z - a.z};
+      const double w[3] = {0.5 * (a.x + b.x) - ctr[0], 0.5 * (a.y + b.y) - ctr[1],
+                           0.5 * (a.z + b.z) - ctr[2]};
+      for (int i = 0; i < 3; ++i) {
+        M.c[i] += t[i];
+        for (int j = 0; j < 3; ++j) M.D[3 * i + j] += t[i] * w[j];
+      }
+      for (int kk = 0; kk < 3; ++kk)
+        for (int i = 0; i < 3; ++i)
+          for (int j = i; j < 3; ++j)
+            M.Q[6 * kk + qidx(i, j)] += t[kk] * ((1.0 / 12.0) * t[i] * t[j] + w[i] * w[j]);
+    }
+  } else {
+    const Mom64 &A = m64[ch.x], &B = m64[ch.y];
+    for (int d = 0; d < 3; ++d) {
+      M.box[d] = fmin(A.box[d], B.box[d]);
+      M.box[3 + d] = fmax(A.box[3 + d], B.box[3 + d]);
+    }
+    double ctr[3];
+    for (int d = 0; d < 3; ++d) ctr[d] = 0.5 * (M.box[3 + d] + M.box[d]);
+    for (int side = 0; side < 2; ++side) {
+      const Mom64 &C = side ? B : A;
+      double r[3];
+      for (int d = 0; d < 3; ++d) r[d] = 0.5 * (C.box[3 + d] + C.box[d]) - ctr[d];
+      for (int i = 0; i < 3; ++i) {
+        M.c[i] += C.c[i];
+        for (int j = 0; j < 3; ++j) M.D[3 * i + j] += C.D[3 * i + j] + C.c[i] * r[j];
+      }
+      for (int kk = 0; kk < 3; ++kk)
+        for (int i = 0; i < 3; ++i)
+          for (int j = i; j < 3; ++j)
+            M.Q[6 * kk + qidx(i, j)] += C.Q[6 * kk + qidx(i, j)] + r[j] * C.D[3 * kk + i] +
+                                        r[i] * C.D[3 * kk + j] + C.c[kk] * (r[i] * r[j]);
+    }
+  }
+  m64[id] = M;
+}
+
+// FP64 construction records -> FP32 node records.
+__global__ __launch_bounds__(256) void bh_finalize(const Mom64 *__restrict__ m64,
+                                                   const int2 *__restrict__ child,
+                                                   const BhRange *__restrict__ range, int32_t n,
+                                                   BhGeo *__restrict__ geo,
+                                                   BhMom *__restrict__ mom) {
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= n) return;
+  const Mom64 &M = m64[id];
+  BhGeo g;
+  double ctr[3], dsq = 0;
+  for (int d = 0; d < 3; ++d) {
+    ctr[d] = 0.5 * (M.box[3 + d] + M.box[d]);
+    const double e = M.box[3 + d] - M.box[d];
+    dsq += e * e;
+  }
+  const float hx = (float)ctr[0], hy = (float)ctr[1], hz = (float)ctr[2];
+  g.cx = hx; g.cy = hy; g.cz = hz;
+  g.lx = (float)(ctr[0] - hx); g.ly = (float)(ctr[1] - hy); g.lz = (float)(ctr[2] - hz);
+  g.rsq = (float)(0.25 * dsq);
+  g.radius = (float)sqrt(0.25 * dsq);
+  g.c0 = child[id].x;
+  g.c1 = child[id].y;
+  g.seg0 = range[id].lo;
+  g.nseg = range[id].hi - range[id].lo;
+  double dn = 0, qn = 0;
+  for (int t = 0; t < 9; ++t) dn += M.D[t] * M.D[t];
+  for (int kk = 0; kk < 3; ++kk)
+    for (int t = 0; t < 6; ++t) {
+      const double v = M.Q[6 * kk + t];
+      qn += ((t == 1 || t == 2 || t == 4) ? 2.0 : 1.0) * v * v;
+    }
+  g.cn = (float)sqrt(M.c[0] * M.c[0] + M.c[1] * M.c[1] + M.c[2] * M.c[2]);
+  g.dn = (float)(sqrt(dn) * 0.57735026918962576451);
+  g.qn = (float)((1.0 / 3.0) * sqrt(qn));
+  g.pad = 0.f;
+  geo[id] = g;
+  BhMom m;
+  for (int t = 0; t < 3; ++t) m.c[t] = (float)M.c[t];
+  for (int t = 0; t < 9; ++t) m.D[t] = (float)M.D[t];
+  for (int t = 0; t < 18; ++t) m.Q[t] = (float)M.Q[t];
+  m.pad[0] = m.pad[1] = 0.f;
+  mom[id] = m;
+}
+
+// ---------------------------------------------------------------- far field
+struct F3 {
+  float x, y, z;
+};
+__device__ __forceinline__ F3 f3(float a, float b, float c) { return F3{a, b, c}; }
+__device__ __forceinline__ float dot3(F3 a, F3 b) { return fmaf(a.x, b.x, fmaf(a.y, b.y, a.z * b.z)); }
+__device__ __forceinline__ F3 cross3(F3 a, F3 b) {
+  return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ F3 add3(F3 a, F3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ F3 sub3(F3 a, F3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ F3 Dv(const float *D, F3 v) {  // D v
+  return f3(fmaf(D[0], v.x, fmaf(D[1], v.y, D[2] * v.z)), fmaf(D[3], v.x, fmaf(D[4], v.y, D[5] * v.z)),
+            fmaf(D[6], v.x, fmaf(D[7], v.y, D[8] * v.z)));
+}
+__device__ __forceinline__ F3 DTv(const float *D, F3 v) {  // D^T v
+  return f3(fmaf(D[0], v.x, fmaf(D[3], v.y, D[6] * v.z)), fmaf(D[1], v.x, fmaf(D[4], v.y, D[7] * v.z)),
+            fmaf(D[2], v.x, fmaf(D[5], v.y, D[8] * v.z)));
+}
+// Q_k v  (row i: sum_j Q_k,ij v_j) from the symmetric packing
+__device__ __forceinline__ F3 Qv(const float *Q, int k, F3 v) {
+  const float *q = Q + 6 * k;
+  return f3(fmaf(q[0], v.x, fmaf(q[1], v.y, q[2] * v.z)), fmaf(q[1], v.x, fmaf(q[3], v.y, q[4] * v.z)),
+            fmaf(q[2], v.x, fmaf(q[4], v.y, q[5] * v.z)));
+}
+__device__ __forceinline__ float Qel(const float *Q, int k, int i, int j) { return Q[6 * k + qidx(i, j)]; }
+
+// Second-order far field between node A (curve 1) and node B (curve 2) with
+// r = ctr_B - ctr_A: returns the linking-number contribution and writes the
+// Eq. 14 error estimate (both already multiplied by 1/(4 pi |r|^3)).
+__device__ float far_field(const BhMom &A, const BhMom &B, const BhGeo &GA, const BhGeo &GB, F3 r,
+                           float dsq, float &err) {
+  const F3 ca = f3(A.c[0], A.c[1], A.c[2]), cb = f3(B.c[0], B.c[1], B.c[2]);
+  const F3 wa = f3(A.D[5] - A.D[7], A.D[6] - A.D[2], A.D[1] - A.D[3]);
+  const F3 wb = f3(B.D[5] - B.D[7], B.D[6] - B.D[2], B.D[1] - B.D[3]);
+  const float ir2 = 1.0f / dsq;
+  float acc = dot3(r, cross3(cb, ca));  // monopole
+  // dipole (Eq. 29)
+  acc -= dot3(ca, wb) + dot3(wa, cb);
+  acc -= 3.0f * ir2 * (dot3(cross3(Dv(B.D, r), ca), r) + dot3(cross3(Dv(A.D, r), cb), r));
+  // quadrupole (Eq. 30)
+  const F3 ta = f3(Qel(A.Q, 0, 0, 0) + Qel(A.Q, 0, 1, 1) + Qel(A.Q, 0, 2, 2),
+                   Qel(A.Q, 1, 0, 0) + Qel(A.Q, 1, 1, 1) + Qel(A.Q, 1, 2, 2),
+                   Qel(A.Q, 2, 0, 0) + Qel(A.Q, 2, 1, 1) + Qel(A.Q, 2, 2, 2));
+  const F3 tb = f3(Qel(B.Q, 0, 0, 0) + Qel(B.Q, 0, 1, 1) + Qel(B.Q, 0, 2, 2),
+                   Qel(B.Q, 1, 0, 0) + Qel(B.Q, 1, 1, 1) + Qel(B.Q, 1, 2, 2),
+                   Qel(B.Q, 2, 0, 0) + Qel(B.Q, 2, 1, 1) + Qel(B.Q, 2, 2, 2));
+  const float s2a = -dot3(r, add3(cross3(ta, cb), cross3(ca, tb)));
+  auto rd = [](const float *X, int a, const float *Y, int b) {
+    return fmaf(X[3 * a], Y[3 * b], fmaf(X[3 * a + 1], Y[3 * b + 1], X[3 * a + 2] * Y[3 * b + 2]));
+  };
+  const float s2b = 2.0f * (r.x * (rd(A.D, 1, B.D, 2) - rd(A.D, 2, B.D, 1)) +
+                            r.y * (rd(A.D, 2, B.D, 0) - rd(A.D, 0, B.D, 2)) +
+                            r.z * (rd(A.D, 0, B.D, 1) - rd(A.D, 1, B.D, 0)));
+  // u_i = eps_ijk (Q_j r)_k ... as (Q1 r)_2 - (Q2 r)_1 etc.
+  const F3 a0 = Qv(A.Q, 0, r), a1 = Qv(A.Q, 1, r), a2 = Qv(A.Q, 2, r);
+  const F3 b0 = Qv(B.Q, 0, r), b1 = Qv(B.Q, 1, r), b2 = Qv(B.Q, 2, r);
+  const F3 ua = f3(a1.z - a2.y, a2.x - a0.z, a0.y - a1.x);
+  const F3 ub = f3(b1.z - b2.y, b2.x - b0.z, b0.y - b1.x);
+  float s3 = dot3(cb, ua) - dot3(ca, ub);
+  s3 += dot3(r, sub3(DTv(A.D, wb), DTv(B.D, wa)));
+  const F3 rra = f3(dot3(r, a0), dot3(r, a1), dot3(r, a2));
+  const F3 rrb = f3(dot3(r, b0), dot3(r, b1), dot3(r, b2));
+  const float s1 = -dot3(r, cross3(cb, rra)) + dot3(r, cross3(ca, rrb)) +
+                   2.0f * dot3(Dv(A.D, r), cross3(r, Dv(B.D, r)));
+  acc -= 1.5f * ir2 * ((s2a + s2b + 2.0f * s3) + 5.0f * ir2 * s1);
+  const float mult = ir2 * __builtin_amdgcn_rsqf(dsq) * kInv4PiF;
+  err = mult * ir2 * (GA.qn * (GA.radius * GB.cn + 3.0f * GB.dn) + GB.qn * (GB.radius * GA.cn + 3.0f * GA.dn));
+  return mult * acc;
+}
+
+// Reference leaf term (compute_link.cpp:576-606) from relative midpoint r and
+// tangents t1, t2: det == 0 handled with copysign as the reference does.
+__device__ __forceinline__ float leaf_term(F3 r, F3 t1, F3 t2) {
+  const float det = dot3(r, cross3(t2, t1));
+  const F3 a = f3(fmaf(0.5f, t1.x - t2.x, r.x), fmaf(0.5f, t1.y - t2.y, r.y), fmaf(0.5f, t1.z - t2.z, r.z));
+  const F3 b = f3(fmaf(-0.5f, t1.x + t2.x, r.x), fmaf(-0.5f, t1.y + t2.y, r.y), fmaf(-0.5f, t1.z + t2.z, r.z));
+  const F3 c = f3(fmaf(0.5f, t2.x - t1.x, r.x), fmaf(0.5f, t2.y - t1.y, r.y), fmaf(0.5f, t2.z - t1.z, r.z));
+  const F3 d = f3(fmaf(0.5f, t2.x + t1.x, r.x), fmaf(0.5f, t2.y + t1.y, r.y), fmaf(0.5f, t2.z + t1.z, r.z));
+  const float la = sqrtf(dot3(a, a)), lb = sqrtf(dot3(b, b)), lc = sqrtf(dot3(c, c)), ld = sqrtf(dot3(d, d));
+  const float ac = dot3(a, c);
+  const float div1 = la * lb * lc + dot3(a, b) * lc + ac * lb + dot3(b, c) * la;
+  const float div2 = la * ld * lc + dot3(a, d) * lc + ac * ld + dot3(d, c) * la;
+  const float xp = fmaf(div1, div2, -det * det);
+  const float s = det == 0.f ? copysignf(1.f, div1) + copysignf(1.f, div2) : div1 + div2;
+  const float yp = det * s;
+  const float sg = det == 0.f ? -copysignf(1.f, div1) : copysignf(1.f, det);
+  return kInv2PiF * atan2f(yp, xp) + ((s < 0.f || (s == 0.f && det < 0.f)) ? sg : 0.f);
+}
+
+// Per-wave FP64 accumulation into per-pair sums: one atomic per wave when
+// the wave's active items share a pair, one per item otherwise.
+__device__ __forceinline__ void pair_add(int pair, bool active, double v, double e,
+                                         double *__restrict__ sum, double *__restrict__ err) {
+  const unsigned long long act = __ballot(active);
+  if (act == 0) return;
+  const int lead = __ffsll((long long)act) - 1;
+  const int p0 = __shfl(pair, lead, 64);
+  const bool uniform = __ballot(active && pair != p0) == 0;
+  if (uniform) {
+    double a = active ? v : 0.0, b = active ? e : 0.0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      a += __shfl_xor(a, off, 64);
+      b += __shfl_xor(b, off, 64);
+    }
+    if ((int)(threadIdx.x & 63) == lead) {
+      atomicAdd(sum + p0, a);
+      if (b != 0.0) atomicAdd(err + p0, b);
+    }
+  } else if (active) {
+    atomicAdd(sum + pair, v);
+    if (e != 0.0) atomicAdd(err + pair, e);
+  }
+}
+
+// Wave-aggregated append of `cnt` slots; returns this lane's first slot.
+__device__ __forceinline__ int64_t wave_append(unsigned long long *counter, int cnt) {
+  int incl = cnt;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(incl, off, 64);
+    if ((int)(threadIdx.x & 63) >= off) incl += y;
+  }
+  const int total = __shfl(incl, 63, 64);
+  unsigned long long base = 0;
+  if ((threadIdx.x & 63) == 63 && total > 0) base = atomicAdd(counter, (unsigned long long)total);
+  base = __shfl(base, 63, 64);
+  return (int64_t)base + incl - cnt;
+}
+
+// One traversal level over a chunk of items.
+__global__ __launch_bounds__(256) void bh_level(
+    const BhItem *__restrict__ items, int64_t n, const BhGeo *__restrict__ geo,
+    const BhMom *__restrict__ mom, const float *__restrict__ beta2, float eps,
+    BhItem *__restrict__ next, unsigned long long *__restrict__ next_count, int64_t next_cap,
+    BhItem *__restrict__ leaves, unsigned long long *__restrict__ leaf_count, int64_t leaf_cap,
+    double *__restrict__ psum, double *__restrict__ perr,
+    unsigned long long *__restrict__ counters) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = t < n;
+  BhItem it = live ? items[t] : BhItem{0, 0, 0, 0};
+  int kind = -1;  // 0 leaf pair, 1 far, 2 split
+  float val = 0.f, e = 0.f;
+  int ca = 0, cb = 0;
+  bool split_a = false;
+  if (live) {
+    const BhGeo A = geo[it.a], B = geo[it.b];
+    const F3 r = f3((B.cx - A.cx) + (B.lx - A.lx), (B.cy - A.cy) + (B.ly - A.ly),
+                    (B.cz - A.cz) + (B.lz - A.lz));
+    const float dsq = dot3(r, r) + eps;
+    const bool la = A.c0 < 0, lb = B.c0 < 0;
+    const float rs = A.radius + B.radius;
+    if (la && lb) {
+      kind = 0;
+    } else if (dsq > rs * rs * beta2[it.pair] + eps) {
+      kind = 1;
+      val = far_field(mom[it.a], mom[it.b], A, B, r, dsq, e);
+    } else {
+      kind = 2;
+      split_a = (!la && !lb) ? !(B.rsq > A.rsq) : lb;
+      if (split_a) {
+        ca = A.c0;
+        cb = A.c1;
+      } else {
+        ca = B.c0;
+        cb = B.c1;
+      }
+    }
+  }
+  // far-field contributions
+  pair_add(it.pair, kind == 1, (double)val, (double)e, psum, perr);
+  // split: two children into the next level
+  {
+    const int64_t s = wave_append(next_count, kind == 2 ? 2 : 0);
+    if (kind == 2 && s + 1 < next_cap) {
+      if (split_a) {
+        next[s] = BhItem{it.pair, ca, it.b, 0};
+        next[s + 1] = BhItem{it.pair, cb, it.b, 0};
+      } else {
+        next[s] = BhItem{it.pair, it.a, ca, 0};
+        next[s + 1] = BhItem{it.pair, it.a, cb, 0};
+      }
+    }
+  }
+  {
+    const int64_t s = wave_append(leaf_count, kind == 0 ? 1 : 0);
+    if (kind == 0 && s < leaf_cap) leaves[s] = it;
+  }
+  // statistics: visits, far evaluations
+  const unsigned long long far_mask = __ballot(kind == 1);
+  const unsigned long long live_mask = __ballot(live);
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(counters + 0, (unsigned long long)__popcll(live_mask));
+    atomicAdd(counters + 1, (unsigned long long)__popcll(far_mask));
+  }
+}
+
+// Exact evaluation of leaf-bucket pairs: every segment pair of the two
+// buckets with the reference leaf term, from FP32 hi/lo vertices.
+__global__ __launch_bounds__(256) void bh_leaf_pairs(
+    const BhItem *__restrict__ items, int64_t n, const BhGeo *__restrict__ geo,
+    const uint32_t *__restrict__ order, const float4 *__restrict__ hi,
+    const float4 *__restrict__ lo, const float4 *__restrict__ ed, double *__restrict__ psum,
+    double *__restrict__ perr, unsigned long long *__restrict__ counters) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = t < n;
+  float acc = 0.f;
+  int npairs = 0;
+  BhItem it = live ? items[t] : BhItem{0, 0, 0, 0};
+  if (live) {
+    const BhGeo A = geo[it.a], B = geo[it.b];
+    for (int i = A.seg0; i < A.seg0 + A.nseg; ++i) {
+      const uint32_t va = order[i];
+      const float4 pa = hi[va], pb = hi[va + 1], qa = lo[va], qb = lo[va + 1], ea = ed[va];
+      for (int j = B.seg0; j < B.seg0 + B.nseg; ++j) {
+        const uint32_t vb = order[j];
+        const float4 ra = hi[vb], rb = hi[vb + 1], sa = lo[vb], sb = lo[vb + 1], eb = ed[vb];
+        // r = m2 - m1 = ((q0 - p0) + (q1 - p1)) / 2, each difference hi/lo exact
+        const F3 r = f3(0.5f * (((ra.x - pa.x) + (sa.x - qa.x)) + ((rb.x - pb.x) + (sb.x - qb.x))),
+                        0.5f * (((ra.y - pa.y) + (sa.y - qa.y)) + ((rb.y - pb.y) + (sb.y - qb.y))),
+                        0.5f * (((ra.z - pa.z) + (sa.z - qa.z)) + ((rb.z - pb.z) + (sb.z - qb.z))));
+        acc += leaf_term(r, f3(ea.x, ea.y, ea.z), f3(eb.x, eb.y, eb.z));
+        ++npairs;
+      }
+    }
+  }
+  pair_add(it.pair, live, (double)acc, 0.0, psum, perr);
+  int tot = npairs;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off, 64);
+  if ((threadIdx.x & 63) == 0 && tot) atomicAdd(counters + 2, (unsigned long long)tot);
+}
+
+__global__ void bh_roots(const int32_t *__restrict__ ra, const int32_t *__restrict__ rb, int64_t n,
+                         BhItem *__restrict__ out) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) out[k] = BhItem{(int32_t)k, ra[k], rb[k], 0};
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- host side
+namespace {
+
+struct Forest {
+  int32_t nnodes = 0;
+  std::vector<int32_t> root;  // per curve (-1 if no tree)
+  DevBuf geo, mom, order;
+};
+
+void build_forest(flk_ctx *ctx, const std::vector<int32_t> &curves_in, int leaf, Forest &F) {
+  Curves &cv = ctx->curves;
+  hipStream_t s = ctx->stream;
+  F.root.assign(cv.ncurves, -1);
+  std::vector<int32_t> cs;
+  for (int32_t c : curves_in)
+    if (cv.n[c] >= 2) cs.push_back(c);
+  const int32_t nc = (int32_t)cs.size();
+  if (nc == 0) return;
+  if (cv.nvert_dev >= (int64_t)UINT32_MAX) throw Error(FLK_ERR_ARG, "too many vertices for Barnes-Hut");
+  std::vector<int64_t> seg_off(nc + 1, 0), vtx_off(nc);
+  std::vector<double> cbox(6 * (size_t)nc);
+  for (int32_t k = 0; k < nc; ++k) {
+    seg_off[k + 1] = seg_off[k] + cv.n[cs[k]];
+    vtx_off[k] = cv.off[cs[k]];
+    for (int d = 0; d < 3; ++d) {
+      cbox[6 * k + d] = (cv.box[6 * cs[k] + d] - cv.origin[d]) * cv.scale;
+      cbox[6 * k + 3 + d] = (cv.box[6 * cs[k] + 3 + d] - cv.origin[d]) * cv.scale;
+    }
+  }
+  const int64_t S = seg_off[nc];
+  if (S >= INT32_MAX / 2) throw Error(FLK_ERR_ARG, "too many Barnes-Hut segments");
+  DevBuf d_seg, d_vtx, d_box, d_k0, d_k1, d_v0, d_tmp;
+  d_seg.reserve(sizeof(int64_t) * (nc + 1));
+  d_vtx.reserve(sizeof(int64_t) * nc);
+  d_box.reserve(sizeof(double) * 6 * nc);
+  d_k0.reserve(sizeof(uint64_t) * S);
+  d_k1.reserve(sizeof(uint64_t) * S);
+  d_v0.reserve(sizeof(uint32_t) * S);
+  F.order.reserve(sizeof(uint32_t) * S);
+  FLK_HIP(hipMemcpyAsync(d_seg.ptr, seg_off.data(), sizeof(int64_t) * (nc + 1), hipMemcpyHostToDevice, s));
+  FLK_HIP(hipMemcpyAsync(d_vtx.ptr, vtx_off.data(), sizeof(int64_t) * nc, hipMemcpyHostToDevice, s));
+  FLK_HIP(hipMemcpyAsync(d_box.ptr, cbox.data(), sizeof(double) * 6 * nc, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(bh_keys, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, s,
+                     cv.dpts.as<double4>(), d_seg.as<int64_t>(), d_vtx.as<int64_t>(),
+                     d_box.as<double>(), nc, S, d_k0.as<uint64_t>(), d_v0.as<uint32_t>());
+  FLK_HIP(hipGetLastError());
+  int end_bit = 48;
+  while ((1LL << (end_bit - 48)) < nc) ++end_bit;
+  size_t tb = 0;
+  FLK_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, d_k0.as<uint64_t>(), d_k1.as<uint64_t>(),
+                                             d_v0.as<uint32_t>(), F.order.as<uint32_t>(), (int)S, 0,
+                                             end_bit, s));
+  d_tmp.reserve(std::max<size_t>(tb, 16));
+  FLK_HIP(hipcub::DeviceRadixSort::SortPairs(d_tmp.ptr, tb, d_k0.as<uint64_t>(), d_k1.as<uint64_t>(),
+                                             d_v0.as<uint32_t>(), F.order.as<uint32_t>(), (int)S, 0,
+                                             end_bit, s));
+  d_k0.release();
+  d_k1.release();
+  d_v0.release();
+
+  // level-by-level node creation
+  std::vector<int32_t> level_base{0}, level_n{nc};
+  std::vector<BhRange> roots(nc);
+  for (int32_t k = 0; k < nc; ++k) roots[k] = BhRange{(int32_t)seg_off[k], (int32_t)seg_off[k + 1]};
+  const int64_t max_nodes = 2 * S + nc;
+  DevBuf d_range, d_child, d_flag, d_scan;
+  d_range.reserve(sizeof(BhRange) * max_nodes);
+  d_child.reserve(sizeof(int2) * max_nodes);
+  d_flag.reserve(sizeof(int32_t) * (S + nc + 1));
+  d_scan.reserve(sizeof(int32_t) * (S + nc + 1));
+  FLK_HIP(hipMemcpyAsync(d_range.ptr, roots.data(), sizeof(BhRange) * nc, hipMemcpyHostToDevice, s));
+  int32_t total = nc;
+  for (int lev = 0;; ++lev) {
+    const int32_t base = level_base[lev], n = level_n[lev];
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(bh_split_flags, dim3(nb), dim3(256), 0, s, d_range.as<BhRange>() + base, n,
+                       leaf, d_flag.as<int32_t>());
+    tb = 0;
+    FLK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, d_flag.as<int32_t>(), d_scan.as<int32_t>(),
+                                             n + 1, s));
+    d_tmp.reserve(std::max<size_t>(tb, 16));
+    // a trailing zero flag makes scan[n] the number of split nodes
+    FLK_HIP(hipMemsetAsync(d_flag.as<int32_t>() + n, 0, sizeof(int32_t), s));
+    FLK_HIP(hipcub::DeviceScan::ExclusiveSum(d_tmp.ptr, tb, d_flag.as<int32_t>(), d_scan.as<int32_t>(),
+                                             n + 1, s));
+    int32_t nsplit = 0;
+    FLK_HIP(hipMemcpyAsync(&nsplit, d_scan.as<int32_t>() + n, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    FLK_HIP(hipStreamSynchronize(s));
+    hipLaunchKernelGGL(bh_split, dim3(nb), dim3(256), 0, s, d_range.as<BhRange>() + base, n, base,
+                       total, d_flag.as<int32_t>(), d_scan.as<int32_t>(),
+                       d_range.as<BhRange>() + total, d_child.as<int2>());
+    FLK_HIP(hipGetLastError());
+    if (nsplit == 0) break;
+    level_base.push_back(total);
+    level_n.push_back(2 * nsplit);
+    total += 2 * nsplit;
+  }
+  F.nnodes = total;
+  DevBuf d_m64;
+  d_m64.reserve(sizeof(Mom64) * total);
+  for (int lev = (int)level_base.size() - 1; lev >= 0; --lev) {
+    const int32_t n = level_n[lev];
+    hipLaunchKernelGGL(bh_moments, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s,
+                       d_range.as<BhRange>() + level_base[lev], n, level_base[lev],
+                       d_child.as<int2>(), F.order.as<uint32_t>(), cv.dpts.as<double4>(),
+                       d_m64.as<Mom64>());
+    FLK_HIP(hipGetLastError());
+  }
+  F.geo.reserve(sizeof(BhGeo) * total);
+  F.mom.reserve(sizeof(BhMom) * total);
+  hipLaunchKernelGGL(bh_finalize, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+                     d_m64.as<Mom64>(), d_child.as<int2>(), d_range.as<BhRange>(), total,
+                     F.geo.as<BhGeo>(), F.mom.as<BhMom>());
+  FLK_HIP(hipGetLastError());
+  FLK_HIP(hipStreamSynchronize(s));
+  for (int32_t k = 0; k < nc; ++k) F.root[cs[k]] = k;  // roots are nodes 0..nc-1
+  d_m64.release();
+  d_range.release();
+  d_child.release();
+}
+
+// Traverses the given pairs (indices into the forest roots) with per-pair
+// beta; adds into psum/perr (device, per pair).
+void traverse(flk_ctx *ctx, Forest &F, const std::vector<int32_t> &ra, const std::vector<int32_t> &rb,
+              const std::vector<float> &beta2, float eps, DevBuf &d_sum, DevBuf &d_err,
+              unsigned long long stats[3]) {
+  hipStream_t s = ctx->stream;
+  const int64_t np = (int64_t)ra.size();
+  if (np == 0) return;
+  DevBuf d_ra, d_rb, d_beta, d_cnt;
+  d_ra.reserve(sizeof(int32_t) * np);
+  d_rb.reserve(sizeof(int32_t) * np);
+  d_beta.reserve(sizeof(float) * np);
+  d_cnt.reserve(sizeof(unsigned long long) * 8);
+  FLK_HIP(hipMemcpyAsync(d_ra.ptr, ra.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice, s));
+  FLK_HIP(hipMemcpyAsync(d_rb.ptr, rb.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice, s));
+  FLK_HIP(hipMemcpyAsync(d_beta.ptr, beta2.data(), sizeof(float) * np, hipMemcpyHostToDevice, s));
+  FLK_HIP(hipMemsetAsync(d_cnt.ptr, 0, sizeof(unsigned long long) * 8, s));
+  unsigned long long *cnt = d_cnt.as<unsigned long long>();  // 0 visits 1 far 2 leaf pairs 3 next 4 leaves
+
+  // LIFO stack of level buffers: each holds items still to process.
+  const int64_t chunk = 1 << 22;
+  struct Level {
+    DevBuf buf;
+    int64_t n = 0;
+  };
+  std::vector<Level> stack(1);
+  stack[0].buf.reserve(sizeof(BhItem) * std::max<int64_t>(np, 1));
+  stack[0].n = np;
+  hipLaunchKernelGGL(bh_roots, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s,
+                     d_ra.as<int32_t>(), d_rb.as<int32_t>(), np, stack[0].buf.as<BhItem>());
+  FLK_HIP(hipGetLastError());
+  DevBuf leaves;
+  leaves.reserve(sizeof(BhItem) * chunk * 2);
+  std::vector<Level> pool;  // reusable buffers
+  size_t depth = 1;
+  while (depth > 0) {
+    Level &top = stack[depth - 1];
+    if (top.n == 0) {
+      --depth;
+      continue;
+    }
+    const int64_t take = std::min<int64_t>(top.n, chunk);
+    top.n -= take;
+    const BhItem *src = top.buf.as<BhItem>() + top.n;
+    if (stack.size() <= depth) stack.resize(depth + 1);
+    Level &nx = stack[depth];
+    nx.buf.reserve(sizeof(BhItem) * 2 * chunk);
+    FLK_HIP(hipMemsetAsync(cnt + 3, 0, 2 * sizeof(unsigned long long), s));
+    hipLaunchKernelGGL(bh_level, dim3((unsigned)((take + 255) / 256)), dim3(256), 0, s, src, take,
+                       F.geo.as<BhGeo>(), F.mom.as<BhMom>(), d_beta.as<float>(), eps,
+                       nx.buf.as<BhItem>(), cnt + 3, 2 * chunk, leaves.as<BhItem>(), cnt + 4,
+                       2 * chunk, d_sum.as<double>(), d_err.as<double>(), cnt);
+    FLK_HIP(hipGetLastError());
+    unsigned long long nn[2] = {0, 0};
+    FLK_HIP(hipMemcpyAsync(nn, cnt + 3, sizeof nn, hipMemcpyDeviceToHost, s));
+    FLK_HIP(hipStreamSynchronize(s));
+    if ((int64_t)nn[0] > 2 * chunk || (int64_t)nn[1] > 2 * chunk)
+      throw Error(FLK_ERR_STATE, "Barnes-Hut level buffer overflow");
+    if (nn[1] > 0) {
+      const Curves &cv = ctx->curves;
+      hipLaunchKernelGGL(bh_leaf_pairs, dim3((unsigned)((nn[1] + 255) / 256)), dim3(256), 0, s,
+                         leaves.as<BhItem>(), (int64_t)nn[1], F.geo.as<BhGeo>(),
+                         F.order.as<uint32_t>(), cv.hi.as<float4>(), cv.lo.as<float4>(),
+                         cv.ed.as<float4>(), d_sum.as<double>(), d_err.as<double>(), cnt);
+      FLK_HIP(hipGetLastError());
+    }
+    nx.n = (int64_t)nn[0];
+    if (nx.n > 0) ++depth;
+  }
+  unsigned long long st[3];
+  FLK_HIP(hipMemcpyAsync(st, cnt, sizeof st, hipMemcpyDeviceToHost, s));
+  FLK_HIP(hipStreamSynchronize(s));
+  for (int k = 0; k < 3; ++k) stats[k] += st[k];
+}
+
+}  // namespace
+
+void bh_run(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, const flk_params &prm,
+            std::vector<double> &lk, std::vector<double> &err, std::vector<uint32_t> &flags) {
+  Curves &cv = ctx->curves;
+  hipStream_t s = ctx->stream;
+  lk.assign(npairs, 0.0);
+  err.assign(npairs, 0.0);
+  flags.assign(npairs, 0u);
+  if (npairs == 0) return;
+  if (prm.criterion == FLK_CRIT_MLP)
+    throw Error(FLK_ERR_STATE, "the learned criterion is not available in this build");
+  // Pairs below the direct threshold (or with a curve too short for a tree)
+  // go to the direct sum; the rest to Barnes–Hut.
+  std::vector<int64_t> ds_idx, bh_idx;
+  std::vector<int32_t> ds_pairs;
+  std::vector<char> used(cv.ncurves, 0);
+  for (int64_t k = 0; k < npairs; ++k) {
+    const int32_t a = pairs[2 * k], b = pairs[2 * k + 1];
+    const int64_t work = (int64_t)cv.n[a] * cv.n[b];
+    if (cv.n[a] < 2 || cv.n[b] < 2 || work < prm.bh_direct_threshold) {
+      ds_idx.push_back(k);
+      ds_pairs.push_back(a);
+      ds_pairs.push_back(b);
+      flags[k] |= FLK_PAIR_DIRECT;
+    } else {
+      bh_idx.push_back(k);
+      used[a] = used[b] = 1;
+    }
+  }
+  if (!ds_idx.empty()) {
+    std::vector<double> v;
+    float ms = 0.f;
+    ds_run(ctx, ds_pairs.data(), (int64_t)ds_idx.size(), prm.fp64 != 0, v, &ms);
+    ctx->stats.ms_ds_kernel += ms;
+    for (size_t t = 0; t < ds_idx.size(); ++t) lk[ds_idx[t]] = v[t];
+  }
+  if (bh_idx.empty()) return;
+  const int leaf = std::max(1, prm.leaf_size);
+  std::vector<int32_t> cs;
+  for (int32_t c = 0; c < cv.ncurves; ++c)
+    if (used[c]) cs.push_back(c);
+  Forest F;
+  auto t0 = std::chrono::steady_clock::now();
+  build_forest(ctx, cs, leaf, F);
+  ctx->stats.ms_bh_build +=
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  const int64_t nb = (int64_t)bh_idx.size();
+  std::vector<int32_t> ra(nb), rb(nb);
+  std::vector<float> beta2(nb);
+  for (int64_t t = 0; t < nb; ++t) {
+    ra[t] = F.root[pairs[2 * bh_idx[t]]];
+    rb[t] = F.root[pairs[2 * bh_idx[t] + 1]];
+    beta2[t] = (float)(prm.beta_init * prm.beta_init);
+  }
+  const float eps = (float)(prm.eps_sq * cv.scale * cv.scale);
+  DevBuf d_sum, d_err;
+  d_sum.reserve(sizeof(double) * nb);
+  d_err.reserve(sizeof(double) * nb);
+  FLK_HIP(hipMemsetAsync(d_sum.ptr, 0, sizeof(double) * nb, s));
+  FLK_HIP(hipMemsetAsync(d_err.ptr, 0, sizeof(double) * nb, s));
+  unsigned long long st[3] = {0, 0, 0};
+  t0 = std::chrono::steady_clock::now();
+  traverse(ctx, F, ra, rb, beta2, eps, d_sum, d_err, st);
+  std::vector<double> v(nb), e(nb);
+  FLK_HIP(hipMemcpyAsync(v.data(), d_sum.ptr, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+  FLK_HIP(hipMemcpyAsync(e.data(), d_err.ptr, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+  FLK_HIP(hipStreamSynchronize(s));
+  // Adaptive beta (reference linking_number_certificate.cpp:165-184): scale
+  // the estimate, target beta = min(beta_init (E / target)^(1/4), limit).
+  std::vector<int64_t> redo;
+  std::vector<int32_t> ra2, rb2;
+  std::vector<float> b2;
+  for (int64_t t = 0; t < nb; ++t) {
+    const double E = e[t] * prm.err_scale;
+    const double target = std::min(std::pow(E / prm.target_error, 0.25) * prm.beta_init, prm.beta_limit);
+    e[t] = E;
+    if (target > prm.beta_init) {
+      redo.push_back(t);
+      ra2.push_back(ra[t]);
+      rb2.push_back(rb[t]);
+      b2.push_back((float)(target * target));
+    }
+  }
+  if (!redo.empty()) {
+    const int64_t nr = (int64_t)redo.size();
+    DevBuf d_sum2, d_err2;
+    d_sum2.reserve(sizeof(double) * nr);
+    d_err2.reserve(sizeof(double) * nr);
+    FLK_HIP(hipMemsetAsync(d_sum2.ptr, 0, sizeof(double) * nr, s));
+    FLK_HIP(hipMemsetAsync(d_err2.ptr, 0, sizeof(double) * nr, s));
+    traverse(ctx, F, ra2, rb2, b2, eps, d_sum2, d_err2, st);
+    std::vector<double> v2(nr), e2(nr);
+    FLK_HIP(hipMemcpyAsync(v2.data(), d_sum2.ptr, sizeof(double) * nr, hipMemcpyDeviceToHost, s));
+    FLK_HIP(hipMemcpyAsync(e2.data(), d_err2.ptr, sizeof(double) * nr, hipMemcpyDeviceToHost, s));
+    FLK_HIP(hipStreamSynchronize(s));
+    for (int64_t t = 0; t < nr; ++t) {
+      v[redo[t]] = v2[t];
+      e[redo[t]] = e2[t] * prm.err_scale;
+      flags[bh_idx[redo[t]]] |= FLK_PAIR_RETRY;
+    }
+    ctx->stats.retries += nr;
+  }
+  ctx->stats.ms_bh_traverse +=
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  ctx->stats.bh_visits += (int64_t)st[0];
+  ctx->stats.bh_far += (int64_t)st[1];
+  ctx->stats.bh_leaf_pairs += (int64_t)st[2];
+  for (int64_t t = 0; t < nb; ++t) {
+    lk[bh_idx[t]] = v[t];
+    err[bh_idx[t]] = e[t];
+  }
+}
+
 }  // namespace flk

@@ -25,15 +25,31 @@ struct Error {
   Error(int32_t c, std::string m) : code(c), msg(std::move(m)) {}
 };
 
-// Device buffer owned by a context; grows, never shrinks.
+// Device allocation that grows on demand and is freed with its owner.
 struct DevBuf {
   void *ptr = nullptr;
   size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : ptr(o.ptr), bytes(o.bytes) {
+    o.ptr = nullptr;
+    o.bytes = 0;
+  }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    if (this != &o) {
+      release();
+      ptr = o.ptr;
+      bytes = o.bytes;
+      o.ptr = nullptr;
+      o.bytes = 0;
+    }
+    return *this;
+  }
+  ~DevBuf() { release(); }
   void reserve(size_t n) {
     if (n <= bytes) return;
-    if (ptr) FLK_HIP(hipFree(ptr));
-    ptr = nullptr;
-    bytes = 0;
+    release();
     if (n == 0) return;
     FLK_HIP(hipMalloc(&ptr, n));
     bytes = n;
