@@ -50,6 +50,10 @@ def main():
     ap.add_argument("--method", default="directsum", choices=["directsum", "barneshut"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="target CPU time of the oracle baseline sample (0 disables)")
+    ap.add_argument("--learned", type=int, default=1,
+                    help="also run learned BH vs GPU direct sum on workload 5 (2x8M)")
+    ap.add_argument("--full-ds", action="store_true",
+                    help="time the full workload-5 direct sum instead of a 1/16 row sample")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
@@ -138,6 +142,7 @@ def main():
                       f"pairs), {dt:.1f} s",
             "certificate_agrees": bool(agree),
         }
+    learned = learned_vs_directsum(eng, flk, workloads, args) if args.learned else None
     out = {
         "metric": "certificate wall time + Gseg-pairs/s; learned BH vs plain GPU directsum, LKs exact",
         "value": round(value, 3),
@@ -166,9 +171,59 @@ def main():
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "learned_bh": learned,
     }
     eng.close()
     print(json.dumps(out), flush=True)
+
+
+BH_BYTES_PER_VISIT = 336  # two 160-byte node records + 16 bytes of queue traffic
+
+
+def learned_vs_directsum(eng, flk, workloads, args):
+    """Workload 5: two interlinked 8M-vertex random-walk loops.  Times the GPU
+    FP32 direct sum (a 1/16 row sample scaled by 16 unless --full-ds: the
+    direct sum's cost is exactly proportional to rows), Barnes–Hut with the
+    reference's beta criterion (adaptive beta as the reference), and
+    Barnes–Hut with the learned criterion."""
+    from fastlinkingnumbers_amd import criterion
+    if not os.path.exists(criterion.DEFAULT_WEIGHTS):
+        log("no learned-criterion weights; skipping the learned comparison")
+        return None
+    curves = workloads.workload(5)
+    a, b = curves
+    res = {"workload": "random_walk_pair 2x8M (rg 200, fp32)", "vertices": [len(a), len(b)]}
+    if args.full_ds:
+        eng.set_curves(curves)
+        r = eng.linking_numbers([[0, 1]], method="directsum", fallback_tol=0.0)
+        res["ds_ms"] = round(r.stats["ms_total"], 1)
+        res["ds_lk"] = float(r.lk[0])
+        res["ds_timing"] = "full"
+    else:
+        eng.set_curves([a[::16], b])
+        eng.linking_numbers([[0, 1]], method="directsum", fallback_tol=0.0)
+        r = eng.linking_numbers([[0, 1]], method="directsum", fallback_tol=0.0)
+        res["ds_ms"] = round(16 * r.stats["ms_total"] * len(a) / (16 * len(a[::16])), 1)
+        res["ds_timing"] = "1/16 row sample (every 16th vertex of curve 1) x 16"
+    eng.set_curves(curves)
+    P, nm = criterion.load_weights()
+    eng.mlp_set_weights(P, nm)
+    for name, kw in (("beta", {}), ("learned", {"criterion": "mlp"})):
+        eng.linking_numbers([[0, 1]], method="barneshut", **kw)  # warm-up
+        r = eng.linking_numbers([[0, 1]], method="barneshut", **kw)
+        st = r.stats
+        res[name] = {"ms": round(st["ms_total"], 1), "lk": float(r.lk[0]),
+                     "off_integer": float(abs(r.lk[0] - round(r.lk[0]))),
+                     "visits": int(st["bh_visits"]), "far": int(st["bh_far"]),
+                     "leaf_pairs": int(st["bh_leaf_pairs"]), "retries": int(st["retries"]),
+                     "fallbacks": int(st["fallbacks"]),
+                     "traverse_GBps": round(st["bh_visits"] * BH_BYTES_PER_VISIT /
+                                            (st["ms_bh_traverse"] * 1e-3) * 1e-9, 1)}
+        log(f"workload 5 {name}: {res[name]}")
+    res["learned_speedup_vs_ds"] = round(res["ds_ms"] / res["learned"]["ms"], 1)
+    res["learned_speedup_vs_beta"] = round(res["beta"]["ms"] / res["learned"]["ms"], 2)
+    res["same_integer"] = round(res["beta"]["lk"]) == round(res["learned"]["lk"])
+    return res
 
 
 if __name__ == "__main__":

@@ -35,27 +35,12 @@
 #include <cmath>
 #include <cstring>
 
+#include "bh_types.h"
 #include "flk_internal.h"
 
 namespace flk {
 
 // ---------------------------------------------------------------- records
-struct BhGeo {      // 64 bytes: what the classification reads
-  float cx, cy, cz, radius;   // centre (hi part) and half box diagonal
-  float lx, ly, lz, rsq;      // centre lo part, radius^2 as 0.25 |diag|^2
-  int32_t c0, c1;             // children (c0 < 0: leaf bucket)
-  int32_t seg0, nseg;         // range in the sorted segment order
-  float cn, dn, qn, pad;      // moment norms
-};
-struct BhMom {      // 128 bytes: read by far-field evaluations
-  float c[3];
-  float D[9];       // D[k*3+j] = int r'_k (r - ctr)_j
-  float Q[18];      // per k: xx xy xz yy yz zz of int r'_k (r-ctr)_i (r-ctr)_j
-  float pad[2];
-};
-struct BhItem {
-  int32_t pair, a, b, pad;
-};
 struct BhRange {    // node range during construction
   int32_t lo, hi;
 };
@@ -397,6 +382,7 @@ __device__ __forceinline__ int64_t wave_append(unsigned long long *counter, int 
 __global__ __launch_bounds__(256) void bh_level(
     const BhItem *__restrict__ items, int64_t n, const BhGeo *__restrict__ geo,
     const BhMom *__restrict__ mom, const float *__restrict__ beta2, float eps,
+    const float *__restrict__ pred, float log_tol,
     BhItem *__restrict__ next, unsigned long long *__restrict__ next_count, int64_t next_cap,
     BhItem *__restrict__ leaves, unsigned long long *__restrict__ leaf_count, int64_t leaf_cap,
     double *__restrict__ psum, double *__restrict__ perr,
@@ -415,11 +401,18 @@ __global__ __launch_bounds__(256) void bh_level(
     const float dsq = dot3(r, r) + eps;
     const bool la = A.c0 < 0, lb = B.c0 < 0;
     const float rs = A.radius + B.radius;
+    // Learned criterion: far field when the spheres are disjoint and the
+    // predicted log10 error is below the tolerance; the squared predicted
+    // error is accumulated (the pair's estimate is the root of the sum, as for
+    // independent signs).  Otherwise the reference's beta test.
+    const bool far = pred ? (dsq > rs * rs + eps && pred[t] < log_tol)
+                          : (dsq > rs * rs * beta2[it.pair] + eps);
     if (la && lb) {
       kind = 0;
-    } else if (dsq > rs * rs * beta2[it.pair] + eps) {
+    } else if (far) {
       kind = 1;
       val = far_field(mom[it.a], mom[it.b], A, B, r, dsq, e);
+      if (pred) e = exp10f(2.0f * pred[t]);
     } else {
       kind = 2;
       split_a = (!la && !lb) ? !(B.rsq > A.rsq) : lb;
@@ -494,6 +487,131 @@ __global__ __launch_bounds__(256) void bh_leaf_pairs(
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off, 64);
   if ((threadIdx.x & 63) == 0 && tot) atomicAdd(counters + 2, (unsigned long long)tot);
+}
+
+// Features of every item of a chunk for the learned criterion.
+__global__ __launch_bounds__(256) void bh_features(const BhItem *__restrict__ items, int64_t n,
+                                                   const BhGeo *__restrict__ geo, float eps,
+                                                   float *__restrict__ feat) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const BhItem it = items[t];
+  const BhGeo A = geo[it.a], B = geo[it.b];
+  const float rx = (B.cx - A.cx) + (B.lx - A.lx), ry = (B.cy - A.cy) + (B.ly - A.ly),
+              rz = (B.cz - A.cz) + (B.lz - A.lz);
+  float f[FLK_MLP_NFEAT];
+  mlp_features(A, B, fmaf(rx, rx, fmaf(ry, ry, rz * rz)) + eps, f);
+#pragma unroll
+  for (int k = 0; k < FLK_MLP_NFEAT; ++k) feat[t * FLK_MLP_NFEAT + k] = f[k];
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z += 0x9e3779b97f4a7c15ULL;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+// Training-sample traversal level: every node pair with disjoint bounding
+// spheres is a candidate sample (kept with probability p when the exact label
+// costs at most max_pairs segment pairs); pairs are split further until they
+// pass beta_stop, so samples cover 1 < |r|/(r1+r2) <= beta_stop at every depth.
+__global__ __launch_bounds__(256) void bh_sample_level(
+    const BhItem *__restrict__ items, int64_t n, const BhGeo *__restrict__ geo, float eps,
+    float stop2, double p, uint64_t seed, int64_t max_pairs, BhItem *__restrict__ next,
+    unsigned long long *__restrict__ next_count, BhItem *__restrict__ samp,
+    unsigned long long *__restrict__ samp_count, int64_t samp_cap) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = t < n;
+  BhItem it = live ? items[t] : BhItem{0, 0, 0, 0};
+  bool split = false, keep = false;
+  int ca = 0, cb = 0;
+  bool split_a = false;
+  if (live) {
+    const BhGeo A = geo[it.a], B = geo[it.b];
+    const float rx = (B.cx - A.cx) + (B.lx - A.lx), ry = (B.cy - A.cy) + (B.ly - A.ly),
+                rz = (B.cz - A.cz) + (B.lz - A.lz);
+    const float dsq = rx * rx + ry * ry + rz * rz + eps;
+    const bool la = A.c0 < 0, lb = B.c0 < 0;
+    const float rs = A.radius + B.radius;
+    if (!(la && lb)) {
+      if (dsq > rs * rs + eps) {
+        const uint64_t h = mix64(seed ^ ((uint64_t)(uint32_t)it.a << 32 | (uint32_t)it.b));
+        keep = (int64_t)A.nseg * B.nseg <= max_pairs && (double)(h >> 11) * 0x1.0p-53 < p;
+      }
+      split = !(dsq > rs * rs * stop2 + eps);
+      if (split) {
+        split_a = (!la && !lb) ? !(B.rsq > A.rsq) : lb;
+        ca = split_a ? A.c0 : B.c0;
+        cb = split_a ? A.c1 : B.c1;
+      }
+    }
+  }
+  const int64_t s = wave_append(next_count, split ? 2 : 0);
+  if (split) {
+    if (split_a) {
+      next[s] = BhItem{0, ca, it.b, 0};
+      next[s + 1] = BhItem{0, cb, it.b, 0};
+    } else {
+      next[s] = BhItem{0, it.a, ca, 0};
+      next[s + 1] = BhItem{0, it.a, cb, 0};
+    }
+  }
+  const int64_t k = wave_append(samp_count, keep ? 1 : 0);
+  if (keep && k < samp_cap) samp[k] = it;
+}
+
+// FP64 one-arctan segment-pair term (reference compute_link.cpp:151-175) of
+// segment p0->p1 against q0->q1, times 4 pi.
+__device__ __forceinline__ double pair_term_x4pi_f64(double4 p0, double4 p1, double4 q0, double4 q1) {
+  const double ax = q0.x - p0.x, ay = q0.y - p0.y, az = q0.z - p0.z;
+  const double bx = q0.x - p1.x, by = q0.y - p1.y, bz = q0.z - p1.z;
+  const double cx = q1.x - p1.x, cy = q1.y - p1.y, cz = q1.z - p1.z;
+  const double dx = q1.x - p0.x, dy = q1.y - p0.y, dz = q1.z - p0.z;
+  const double la = sqrt(ax * ax + ay * ay + az * az), lb = sqrt(bx * bx + by * by + bz * bz);
+  const double lc = sqrt(cx * cx + cy * cy + cz * cz), ld = sqrt(dx * dx + dy * dy + dz * dz);
+  const double ac = cx * ax + cy * ay + cz * az;
+  const double div1 = la * lb * lc + (ax * bx + ay * by + az * bz) * lc + ac * lb + (cx * bx + cy * by + cz * bz) * la;
+  const double div2 = la * ld * lc + (ax * dx + ay * dy + az * dz) * lc + ac * ld + (cx * dx + cy * dy + cz * dz) * la;
+  const double det = ax * (by * cz - bz * cy) + ay * (bz * cx - bx * cz) + az * (bx * cy - by * cx);
+  const double xp = div1 * div2 - det * det, yp = det * (div1 + div2);
+  double v = 2.0 * atan2(yp, xp);
+  if (yp * det < 0) v += (det > 0 ? 1.0 : -1.0) * 12.566370614359172953850573533118;
+  return v;
+}
+
+// One wave per sample: exact FP64 sum over the two nodes' segment pairs, the
+// FP32 far field, features and the label log10 |exact - far field| (>= -14).
+__global__ __launch_bounds__(256) void bh_sample_labels(
+    const BhItem *__restrict__ samp, int64_t n, const BhGeo *__restrict__ geo,
+    const BhMom *__restrict__ mom, const uint32_t *__restrict__ order,
+    const double4 *__restrict__ pts, float eps, float *__restrict__ x, float *__restrict__ y) {
+  const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int l = threadIdx.x & 63;
+  if (w >= n) return;
+  const BhItem it = samp[w];
+  const BhGeo A = geo[it.a], B = geo[it.b];
+  double acc = 0.0;
+  const int64_t total = (int64_t)A.nseg * B.nseg;
+  for (int64_t k = l; k < total; k += 64) {
+    const uint32_t va = order[A.seg0 + (int)(k / B.nseg)], vb = order[B.seg0 + (int)(k % B.nseg)];
+    acc += pair_term_x4pi_f64(pts[va], pts[va + 1], pts[vb], pts[vb + 1]);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if (l == 0) {
+    const double exact = acc * 0.07957747154594766788444188168625718;
+    const F3 r = f3((B.cx - A.cx) + (B.lx - A.lx), (B.cy - A.cy) + (B.ly - A.ly),
+                    (B.cz - A.cz) + (B.lz - A.lz));
+    const float dsq = dot3(r, r) + eps;
+    float e;
+    const float ff = far_field(mom[it.a], mom[it.b], A, B, r, dsq, e);
+    const double d = fabs(exact - (double)ff);
+    y[w] = (float)fmax(log10(d), -14.0);
+    float f[FLK_MLP_NFEAT];
+    mlp_features(A, B, dsq, f);
+    for (int k2 = 0; k2 < FLK_MLP_NFEAT; ++k2) x[w * FLK_MLP_NFEAT + k2] = f[k2];
+  }
 }
 
 __global__ void bh_roots(const int32_t *__restrict__ ra, const int32_t *__restrict__ rb, int64_t n,
@@ -629,7 +747,7 @@ void build_forest(flk_ctx *ctx, const std::vector<int32_t> &curves_in, int leaf,
 // beta; adds into psum/perr (device, per pair).
 void traverse(flk_ctx *ctx, Forest &F, const std::vector<int32_t> &ra, const std::vector<int32_t> &rb,
               const std::vector<float> &beta2, float eps, DevBuf &d_sum, DevBuf &d_err,
-              unsigned long long stats[3]) {
+              unsigned long long stats[3], bool use_mlp = false, float log_tol = 0.f) {
   hipStream_t s = ctx->stream;
   const int64_t np = (int64_t)ra.size();
   if (np == 0) return;
@@ -656,8 +774,13 @@ void traverse(flk_ctx *ctx, Forest &F, const std::vector<int32_t> &ra, const std
   hipLaunchKernelGGL(bh_roots, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s,
                      d_ra.as<int32_t>(), d_rb.as<int32_t>(), np, stack[0].buf.as<BhItem>());
   FLK_HIP(hipGetLastError());
-  DevBuf leaves;
+  DevBuf leaves, feat, pred;
   leaves.reserve(sizeof(BhItem) * chunk * 2);
+  if (use_mlp) {
+    mlp_upload(ctx);
+    feat.reserve(sizeof(float) * FLK_MLP_NFEAT * chunk);
+    pred.reserve(sizeof(float) * chunk);
+  }
   std::vector<Level> pool;  // reusable buffers
   size_t depth = 1;
   while (depth > 0) {
@@ -673,8 +796,15 @@ void traverse(flk_ctx *ctx, Forest &F, const std::vector<int32_t> &ra, const std
     Level &nx = stack[depth];
     nx.buf.reserve(sizeof(BhItem) * 2 * chunk);
     FLK_HIP(hipMemsetAsync(cnt + 3, 0, 2 * sizeof(unsigned long long), s));
+    if (use_mlp) {
+      hipLaunchKernelGGL(bh_features, dim3((unsigned)((take + 255) / 256)), dim3(256), 0, s, src, take,
+                         F.geo.as<BhGeo>(), eps, feat.as<float>());
+      FLK_HIP(hipGetLastError());
+      mlp_forward_device(ctx, feat.as<float>(), take, pred.as<float>());
+    }
     hipLaunchKernelGGL(bh_level, dim3((unsigned)((take + 255) / 256)), dim3(256), 0, s, src, take,
                        F.geo.as<BhGeo>(), F.mom.as<BhMom>(), d_beta.as<float>(), eps,
+                       use_mlp ? pred.as<float>() : nullptr, log_tol,
                        nx.buf.as<BhItem>(), cnt + 3, 2 * chunk, leaves.as<BhItem>(), cnt + 4,
                        2 * chunk, d_sum.as<double>(), d_err.as<double>(), cnt);
     FLK_HIP(hipGetLastError());
@@ -702,6 +832,69 @@ void traverse(flk_ctx *ctx, Forest &F, const std::vector<int32_t> &ra, const std
 
 }  // namespace
 
+int64_t mlp_samples(flk_ctx *ctx, int32_t a, int32_t b, int64_t cap, uint64_t seed, float *x,
+                    float *y) {
+  Curves &cv = ctx->curves;
+  hipStream_t s = ctx->stream;
+  if (cap <= 0) return 0;
+  if (cv.n[a] < 2 || cv.n[b] < 2) return 0;
+  Forest F;
+  std::vector<int32_t> cs{std::min(a, b), std::max(a, b)};
+  if (a == b) cs.resize(1);
+  build_forest(ctx, cs, 1, F);
+  const float eps = (float)(1e-24 * cv.scale * cv.scale);
+  const float stop2 = 36.0f;                // samples up to |r| = 6 (r1 + r2)
+  const int64_t max_pairs = 1 << 20;        // exact-label cost cap per sample
+  const int64_t chunk = 1 << 22;
+  DevBuf d_cnt, d_samp, d_a, d_b;
+  d_cnt.reserve(sizeof(unsigned long long) * 4);
+  d_a.reserve(sizeof(BhItem) * 2 * chunk);
+  d_b.reserve(sizeof(BhItem) * 2 * chunk);
+  d_samp.reserve(sizeof(BhItem) * cap);
+  unsigned long long *cnt = d_cnt.as<unsigned long long>();
+  auto run = [&](double p, int64_t samp_cap) -> int64_t {
+    // breadth-first over levels; a level wider than the chunk is thinned by
+    // keeping a random subset (the samples stay spread over all depths)
+    FLK_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * 4, s));
+    BhItem root{0, F.root[a], F.root[b], 0};
+    FLK_HIP(hipMemcpyAsync(d_a.ptr, &root, sizeof root, hipMemcpyHostToDevice, s));
+    int64_t n = 1;
+    DevBuf *cur = &d_a, *nxt = &d_b;
+    while (n > 0) {
+      const int64_t take = std::min(n, chunk);
+      FLK_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), s));
+      hipLaunchKernelGGL(bh_sample_level, dim3((unsigned)((take + 255) / 256)), dim3(256), 0, s,
+                         cur->as<BhItem>(), take, F.geo.as<BhGeo>(), eps, stop2, p, seed, max_pairs,
+                         nxt->as<BhItem>(), cnt, d_samp.as<BhItem>(), cnt + 1, samp_cap);
+      FLK_HIP(hipGetLastError());
+      unsigned long long nn = 0;
+      FLK_HIP(hipMemcpyAsync(&nn, cnt, sizeof nn, hipMemcpyDeviceToHost, s));
+      FLK_HIP(hipStreamSynchronize(s));
+      n = (int64_t)nn;
+      std::swap(cur, nxt);
+    }
+    unsigned long long ns = 0;
+    FLK_HIP(hipMemcpyAsync(&ns, cnt + 1, sizeof ns, hipMemcpyDeviceToHost, s));
+    FLK_HIP(hipStreamSynchronize(s));
+    return (int64_t)ns;
+  };
+  const int64_t eligible = run(1.0, 0);
+  if (eligible == 0) return 0;
+  const double p = std::min(1.0, 1.05 * (double)cap / (double)eligible);
+  const int64_t got = std::min(run(p, cap), cap);
+  DevBuf dx, dy;
+  dx.reserve(sizeof(float) * FLK_MLP_NFEAT * got);
+  dy.reserve(sizeof(float) * got);
+  hipLaunchKernelGGL(bh_sample_labels, dim3((unsigned)((got * 64 + 255) / 256)), dim3(256), 0, s,
+                     d_samp.as<BhItem>(), got, F.geo.as<BhGeo>(), F.mom.as<BhMom>(),
+                     F.order.as<uint32_t>(), cv.dpts.as<double4>(), eps, dx.as<float>(), dy.as<float>());
+  FLK_HIP(hipGetLastError());
+  FLK_HIP(hipMemcpyAsync(x, dx.ptr, sizeof(float) * FLK_MLP_NFEAT * got, hipMemcpyDeviceToHost, s));
+  FLK_HIP(hipMemcpyAsync(y, dy.ptr, sizeof(float) * got, hipMemcpyDeviceToHost, s));
+  FLK_HIP(hipStreamSynchronize(s));
+  return got;
+}
+
 void bh_run(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, const flk_params &prm,
             std::vector<double> &lk, std::vector<double> &err, std::vector<uint32_t> &flags) {
   Curves &cv = ctx->curves;
@@ -710,8 +903,9 @@ void bh_run(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, const flk_params
   err.assign(npairs, 0.0);
   flags.assign(npairs, 0u);
   if (npairs == 0) return;
-  if (prm.criterion == FLK_CRIT_MLP)
-    throw Error(FLK_ERR_STATE, "the learned criterion is not available in this build");
+  const bool use_mlp = prm.criterion == FLK_CRIT_MLP;
+  if (use_mlp && !ctx->mlp.loaded)
+    throw Error(FLK_ERR_STATE, "the learned criterion needs weights (flk_mlp_set_weights)");
   // Pairs below the direct threshold (or with a curve too short for a tree)
   // go to the direct sum; the rest to Barnes–Hut.
   std::vector<int64_t> ds_idx, bh_idx;
@@ -763,7 +957,7 @@ void bh_run(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, const flk_params
   FLK_HIP(hipMemsetAsync(d_err.ptr, 0, sizeof(double) * nb, s));
   unsigned long long st[3] = {0, 0, 0};
   t0 = std::chrono::steady_clock::now();
-  traverse(ctx, F, ra, rb, beta2, eps, d_sum, d_err, st);
+  traverse(ctx, F, ra, rb, beta2, eps, d_sum, d_err, st, use_mlp, (float)prm.mlp_log_tol);
   std::vector<double> v(nb), e(nb);
   FLK_HIP(hipMemcpyAsync(v.data(), d_sum.ptr, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
   FLK_HIP(hipMemcpyAsync(e.data(), d_err.ptr, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
@@ -773,7 +967,7 @@ void bh_run(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, const flk_params
   std::vector<int64_t> redo;
   std::vector<int32_t> ra2, rb2;
   std::vector<float> b2;
-  for (int64_t t = 0; t < nb; ++t) {
+  for (int64_t t = 0; t < nb && !use_mlp; ++t) {
     const double E = e[t] * prm.err_scale;
     const double target = std::min(std::pow(E / prm.target_error, 0.25) * prm.beta_init, prm.beta_limit);
     e[t] = E;
@@ -784,6 +978,19 @@ void bh_run(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, const flk_params
       b2.push_back((float)(target * target));
     }
   }
+  // Learned criterion: the root-sum-square of the predicted errors is the
+  // estimate (it tracks the observed error within a few times, where the
+  // reference's summed Eq. 14 bound is scaled by 0.05); a pair whose estimate
+  // exceeds a tenth of the target is re-run one decade stricter.
+  for (int64_t t = 0; t < nb && use_mlp; ++t) {
+    e[t] = std::sqrt(e[t]);
+    if (e[t] > 0.1 * prm.target_error) {
+      redo.push_back(t);
+      ra2.push_back(ra[t]);
+      rb2.push_back(rb[t]);
+      b2.push_back((float)(prm.beta_init * prm.beta_init));
+    }
+  }
   if (!redo.empty()) {
     const int64_t nr = (int64_t)redo.size();
     DevBuf d_sum2, d_err2;
@@ -791,14 +998,14 @@ void bh_run(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, const flk_params
     d_err2.reserve(sizeof(double) * nr);
     FLK_HIP(hipMemsetAsync(d_sum2.ptr, 0, sizeof(double) * nr, s));
     FLK_HIP(hipMemsetAsync(d_err2.ptr, 0, sizeof(double) * nr, s));
-    traverse(ctx, F, ra2, rb2, b2, eps, d_sum2, d_err2, st);
+    traverse(ctx, F, ra2, rb2, b2, eps, d_sum2, d_err2, st, use_mlp, (float)prm.mlp_log_tol - 1.0f);
     std::vector<double> v2(nr), e2(nr);
     FLK_HIP(hipMemcpyAsync(v2.data(), d_sum2.ptr, sizeof(double) * nr, hipMemcpyDeviceToHost, s));
     FLK_HIP(hipMemcpyAsync(e2.data(), d_err2.ptr, sizeof(double) * nr, hipMemcpyDeviceToHost, s));
     FLK_HIP(hipStreamSynchronize(s));
     for (int64_t t = 0; t < nr; ++t) {
       v[redo[t]] = v2[t];
-      e[redo[t]] = e2[t] * prm.err_scale;
+      e[redo[t]] = use_mlp ? std::sqrt(e2[t]) : e2[t] * prm.err_scale;
       flags[bh_idx[redo[t]]] |= FLK_PAIR_RETRY;
     }
     ctx->stats.retries += nr;

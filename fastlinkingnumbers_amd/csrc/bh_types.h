@@ -1,0 +1,51 @@
+// Barnes–Hut node records and the learned criterion's node-pair features.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace flk {
+
+struct BhGeo {      // 64 bytes: what the classification reads
+  float cx, cy, cz, radius;   // centre (hi part) and half box diagonal
+  float lx, ly, lz, rsq;      // centre lo part, radius^2 as 0.25 |diag|^2
+  int32_t c0, c1;             // children (c0 < 0: leaf bucket)
+  int32_t seg0, nseg;         // range in the sorted segment order
+  float cn, dn, qn, pad;      // moment norms |c|, |D|_F/sqrt3, sqrt(sum |Q_k|^2)/3
+};
+struct BhMom {      // 128 bytes: read by far-field evaluations
+  float c[3];
+  float D[9];       // D[k*3+j] = int r'_k (r - ctr)_j
+  float Q[18];      // per k: xx xy xz yy yz zz of int r'_k (r-ctr)_i (r-ctr)_j
+  float pad[2];
+};
+struct BhItem {
+  int32_t pair, a, b, pad;
+};
+
+// 12 dimensionless features of a node pair at centre distance d: both radii,
+// their sum, and the monopole/dipole/quadrupole norms scaled by d^-1, d^-2,
+// d^-3, the segment counts and the Eq. 14 error estimate of the reference
+// (compute_link.cpp:517-522) in linking-number units (all natural logs,
+// floored at log 1e-12).  Scale-free, so a network trained on one model size
+// applies to another.
+__device__ __forceinline__ void mlp_features(const BhGeo &A, const BhGeo &B, float dsq, float *f) {
+  const float id = __builtin_amdgcn_rsqf(dsq), id2 = id * id;
+  const float tiny = 1e-12f;
+  f[0] = __logf(A.radius * id + tiny);
+  f[1] = __logf(B.radius * id + tiny);
+  f[2] = __logf((A.radius + B.radius) * id + tiny);
+  f[3] = __logf(A.cn * id + tiny);
+  f[4] = __logf(A.dn * id2 + tiny);
+  f[5] = __logf(A.qn * id2 * id + tiny);
+  f[6] = __logf(B.cn * id + tiny);
+  f[7] = __logf(B.dn * id2 + tiny);
+  f[8] = __logf(B.qn * id2 * id + tiny);
+  f[9] = __logf((float)A.nseg);
+  f[10] = __logf((float)B.nseg);
+  const float est = 0.0795774715f * id2 * id2 * id *
+                    (A.qn * (A.radius * B.cn + 3.0f * B.dn) + B.qn * (B.radius * A.cn + 3.0f * A.dn));
+  f[11] = __logf(est + tiny);
+}
+
+}  // namespace flk

@@ -48,7 +48,7 @@ int32_t flk_default_params(flk_params *p) {
   p->bh_direct_threshold = 0;
   p->leaf_size = 1;
   p->fp64 = 0;
-  p->mlp_log_tol = -7.0;
+  p->mlp_log_tol = -6.0;
   return FLK_OK;
 }
 

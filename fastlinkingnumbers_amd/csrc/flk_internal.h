@@ -157,6 +157,9 @@ void convert_vertices_launch(hipStream_t s, const double4 *d, int64_t n, float4 
 int64_t mlp_samples(flk_ctx *ctx, int32_t a, int32_t b, int64_t cap, uint64_t seed, float *x,
                     float *y);
 void mlp_forward(flk_ctx *ctx, const float *x, int64_t n, float *y);
+void mlp_upload(flk_ctx *ctx);
+// Forward pass on device buffers (features n x 12, predictions n).
+void mlp_forward_device(flk_ctx *ctx, const float *dx, int64_t n, float *dy);
 float mlp_gradient(flk_ctx *ctx, const float *x, const float *y, int64_t n, float *grad);
 float mlp_train(flk_ctx *ctx, const float *x, const float *y, int64_t n, int epochs, int batch,
                 float lr, uint64_t seed);

@@ -3,8 +3,11 @@
 // the linking-number stage on the MI355X engine, rounding and file I/O.
 #include "linking_number_certificate.h"
 
+#include <dlfcn.h>
+
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <fstream>
 #include <iostream>
 #include <map>
@@ -40,6 +43,30 @@ void engine_check(flk_ctx *ctx, int32_t rc, const char *what) {
 
 // ---------------------------------------------------------------- helpers
 namespace {
+
+// Learned-criterion weight file: "FLKMLP1\0", 5,057 + 24 float32 (little
+// endian).  The default lives beside the package's lib/ directory.
+void load_mlp_weights(flk_ctx *ctx, std::string path) {
+  if (path.empty()) {
+    Dl_info info;
+    if (dladdr((void *)&load_mlp_weights, &info) && info.dli_fname) {
+      std::string lib = info.dli_fname;
+      const size_t slash = lib.rfind('/');
+      path = (slash == std::string::npos ? std::string(".") : lib.substr(0, slash)) +
+             "/../mlp_criterion.bin";
+    }
+  }
+  std::ifstream f(path, std::ios::binary);
+  if (!f) throw std::runtime_error("cannot open MLP weight file " + path);
+  char magic[8];
+  std::vector<float> buf(FLK_MLP_NPARAM + 2 * FLK_MLP_NFEAT);
+  f.read(magic, 8);
+  f.read(reinterpret_cast<char *>(buf.data()), sizeof(float) * buf.size());
+  if (!f || std::memcmp(magic, "FLKMLP1\0", 8) != 0)
+    throw std::runtime_error("not an MLP weight file: " + path);
+  engine_check(ctx, flk_mlp_set_weights(ctx, buf.data(), buf.data() + FLK_MLP_NPARAM),
+               "flk_mlp_set_weights");
+}
 
 // Uploads the start points of every curve (the polyline the linking-number
 // stage integrates; reference compute_link.cpp:204-205).
@@ -167,6 +194,8 @@ void LinkingNumberCertificate::ComputeFromModel(const Model &model, bool force_d
   prm.fallback_tol = opt.fallback_tol;
   prm.bh_direct_threshold = opt.bh_direct_threshold;
   prm.leaf_size = opt.leaf_size;
+  prm.mlp_log_tol = opt.mlp_log_tol;
+  if (opt.learned_criterion && !force_direct_sum) load_mlp_weights(ctx, opt.mlp_weights);
   if (v) std::cout << (force_direct_sum ? "Starting Direct Summation." : "Starting Barnes–Hut.") << std::endl;
   std::vector<int32_t> flat(2 * pairs.size());
   for (size_t k = 0; k < pairs.size(); ++k) {

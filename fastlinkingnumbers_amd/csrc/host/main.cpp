@@ -32,6 +32,8 @@ int main(int argc, char **argv) {
   F.add("bh_direct_threshold", "0", "Barnes-Hut pairs with n1*n2 below this use direct summation");
   F.add("leaf_size", "1", "Barnes-Hut leaf bucket size");
   F.add("device", "0", "GPU device index");
+  F.add("mlp_log_tol", "-6", "learned criterion: accept far field below this predicted log10 error");
+  F.add("mlp_weights", "", "learned criterion weight file (default: the package's mlp_criterion.bin)");
   try {
     F.parse(argc, argv);
   } catch (const std::exception &e) {
@@ -59,6 +61,8 @@ int main(int argc, char **argv) {
   opt.bh_direct_threshold = (int64_t)F.num("bh_direct_threshold");
   opt.leaf_size = (int)F.num("leaf_size");
   opt.device = (int)F.num("device");
+  opt.mlp_log_tol = F.num("mlp_log_tol");
+  opt.mlp_weights = F.str("mlp_weights");
   LinkingNumberCertificate certificate;
   {
     std::cout << "Computing the certificate using the " << (ds ? "Direct Summation" : "Barnes–Hut")

@@ -27,6 +27,8 @@ struct EngineOptions {
   int leaf_size = 1;               // Barnes–Hut leaf bucket size
   int device = 0;
   bool verbose = true;             // print the reference's progress lines
+  double mlp_log_tol = -6.0;       // learned criterion: far field below this log10 error
+  std::string mlp_weights;         // weight file; empty = mlp_criterion.bin of the package
 };
 
 class LinkingNumberCertificate {

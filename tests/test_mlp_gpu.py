@@ -1,0 +1,76 @@
+"""GPU checks of the bf16-MFMA MLP kernels against the numpy reference, and
+of the learned criterion inside Barnes–Hut.
+
+Tolerances: forward within 2e-3 (1 + |y|) of the numpy model that rounds the
+same operands to bf16; the gradient within 2e-3 max|g| of that model, and at
+cosine similarity > 0.999 with the exact float64 gradient."""
+import numpy as np
+import pytest
+
+import mlp_ref as M
+
+flk = pytest.importorskip("fastlinkingnumbers_amd")
+from fastlinkingnumbers_amd import workloads as W, criterion  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(engine, seed=3):
+    g = np.random.default_rng(seed)
+    P = M.init_params(seed)
+    P[768:832] = g.normal(scale=0.1, size=64)
+    P[4928:4992] = g.normal(scale=0.1, size=64)
+    P[5056] = -2.0
+    norm = np.concatenate([g.normal(size=12), g.uniform(0.5, 2, size=12)]).astype(np.float32)
+    engine.mlp_set_weights(P, norm)
+    return P, norm, g
+
+
+def test_mlp_forward_matches_numpy(engine):
+    P, norm, g = _setup(engine)
+    X = (g.normal(size=(203, 12)) * norm[12:] + norm[:12]).astype(np.float32)
+    y = engine.mlp_forward(X)
+    ref, _ = M.forward(P, norm.astype(np.float64), X, emulate=True)
+    err = np.abs(y - ref)
+    print("forward max err", err.max())
+    assert np.all(err <= 2e-3 * (1 + np.abs(ref)))
+
+
+def test_mlp_gradient_matches_cpu(engine):
+    P, norm, g = _setup(engine, 4)
+    X = (g.normal(size=(517, 12)) * norm[12:] + norm[:12]).astype(np.float32)
+    T = g.normal(size=517).astype(np.float32)
+    grad, loss = engine.mlp_gradient(X, T)
+    l_em, g_em = M.loss_grad(P, norm.astype(np.float64), X, T, emulate=True)
+    l64, g64 = M.loss_grad(P, norm.astype(np.float64), X, T, emulate=False)
+    cos = float(grad @ g64 / np.linalg.norm(grad) / np.linalg.norm(g64))
+    print("loss", loss, l_em, l64, "max|g-g_em|", np.abs(grad - g_em).max(), "max|g|", np.abs(g_em).max(), "cos64", cos)
+    assert abs(loss - l_em) <= 1e-3 * (1 + l_em)
+    assert np.abs(grad - g_em).max() <= 2e-3 * np.abs(g_em).max()
+    assert cos > 0.999
+
+
+def test_mlp_train_fits_smooth_target(engine):
+    g = np.random.default_rng(7)
+    X = g.normal(size=(40000, 12)).astype(np.float32)
+    T = (np.tanh(X[:, 0]) + 0.5 * X[:, 1] * X[:, 2] - 0.3 * X[:, 3] ** 2).astype(np.float32)
+    norm = np.concatenate([np.zeros(12), np.ones(12)]).astype(np.float32)
+    engine.mlp_set_weights(criterion.init_params(1), norm)
+    base = float(np.mean((T - T.mean()) ** 2))
+    loss = engine.mlp_train(X, T, epochs=20, batch=2048, lr=3e-3, seed=1)
+    pred = engine.mlp_forward(X)
+    print("baseline mse", base, "train mse", loss, "eval mse", np.mean((pred - T) ** 2))
+    assert np.mean((pred - T) ** 2) < 0.1 * base
+
+
+def test_samples_labels(engine):
+    curves = W.random_walk_pair(seed=9, n=20000, rg=30.0, workload=5)
+    engine.set_curves(curves)
+    x, y = engine.mlp_samples(0, 1, 20000, seed=1)
+    print("samples", len(x), "label range", y.min(), y.max())
+    assert 1000 < len(x) <= 20000
+    assert np.all(np.isfinite(x)) and np.all(np.isfinite(y))
+    assert y.min() >= -14 and y.max() < 1
+    # the error grows as the nodes get closer relative to their size
+    near = x[:, 2] > np.log(0.5)      # (r1 + r2) / |r| > 0.5
+    assert y[near].mean() > y[~near].mean()
