@@ -80,3 +80,11 @@ def test_polyline_no_discretize(tmp_path):
         lines = out.read_text().splitlines()
         assert lines[0] == "25"
         assert [tuple(int(x) for x in l.split(",")) for l in lines[1:]] == ref
+
+
+def test_learned_criterion_cli(tmp_path):
+    out = tmp_path / "kb.txt"
+    stdout = _run([f"--input={os.path.join(DATA, 'knittubebroken.bcc')}", f"--output={out}",
+                   "--criterion=mlp"])
+    assert out.read_text() == "39\n5,6,-2\n"
+    assert "Mean Off-Integer Error" in stdout

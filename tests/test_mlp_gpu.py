@@ -74,3 +74,29 @@ def test_samples_labels(engine):
     # the error grows as the nodes get closer relative to their size
     near = x[:, 2] > np.log(0.5)      # (r1 + r2) / |r| > 0.5
     assert y[near].mean() > y[~near].mean()
+
+
+def test_learned_criterion_gain(engine):
+    """Ablation on a held-out pair of interlinked 1M-vertex random walks (seeds
+    not used in training): the trained criterion meets the certificate bar
+    with fewer node-pair visits and less time than the beta criterion with the
+    reference's adaptive beta."""
+    P, nm = criterion.load_weights()
+    engine.mlp_set_weights(P, nm)
+    curves = W.random_walk_pair(seed=777, n=1_000_000, rg=70.7, workload=5)
+    engine.set_curves(curves)
+    ref = engine.linking_numbers([[0, 1]], method="directsum", fp64=1).lk[0]
+    out = {}
+    for name, kw in (("beta", {}), ("learned", {"criterion": "mlp"})):
+        engine.linking_numbers([[0, 1]], method="barneshut", fallback_tol=0.0, **kw)
+        r = engine.linking_numbers([[0, 1]], method="barneshut", fallback_tol=0.0, **kw)
+        out[name] = (r.lk[0], r.stats["ms_total"], r.stats["bh_visits"], r.err[0])
+        print(name, "lk", r.lk[0], "err", abs(r.lk[0] - ref), "ms", r.stats["ms_total"],
+              "visits", r.stats["bh_visits"], "estimate", r.err[0])
+    for lk, _, _, _ in out.values():
+        assert round(lk) == round(ref) and abs(lk - ref) < 0.05
+    assert out["learned"][2] < out["beta"][2]
+    assert out["learned"][1] < out["beta"][1]
+    # the learned estimate is within a factor 10 of the observed error
+    obs = abs(out["learned"][0] - ref)
+    assert out["learned"][3] > obs / 10
