@@ -105,7 +105,6 @@ __global__ __launch_bounds__(256) void ds_tile_f32(
     float px[R + 1], py[R + 1], pz[R + 1];
     float lx[R + 1], ly[R + 1], lz[R + 1];
     float ex[R], ey[R], ez[R];
-    bool valid[R];
 #pragma unroll
     for (int k = 0; k <= R; ++k) {
       const int64_t v = pr.off_r + min(rbase + k, pr.n_r);
@@ -118,23 +117,23 @@ __global__ __launch_bounds__(256) void ds_tile_f32(
         lx[k] = ly[k] = lz[k] = 0.f;
       }
     }
+    // Rows past the curve end become zero-length segments at the closing
+    // vertex with a zero edge: det = 0, y' = +-0, x' >= 0, so they add +-0.
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-      const int64_t v = pr.off_r + min(rbase + k, pr.n_r);
-      const float4 e = ed[v];
-      ex[k] = e.x; ey[k] = e.y; ez[k] = e.z;
-      valid[k] = rbase + k < pr.n_r;
+      const bool valid = rbase + k < pr.n_r;
+      const float4 e = ed[pr.off_r + min(rbase + k, pr.n_r)];
+      ex[k] = valid ? e.x : 0.f;
+      ey[k] = valid ? e.y : 0.f;
+      ez[k] = valid ? e.z : 0.f;
     }
 
-    // Relative vectors of one grid column: G(k, j) = q_j - p_k, its norm, and
-    // the column dot G(k, j) . G(k+1, j).
-    auto column = [&](int64_t v, float (&gx)[R + 1], float (&gy)[R + 1], float (&gz)[R + 1],
-                      float (&gl)[R + 1]) {
-      const float4 q = hi[v];
+    // Relative vectors of one grid column: G(k, j) = q_j - p_k and its norm.
+    auto column = [&](const float4 &q, const float4 &ql, float (&gx)[R + 1], float (&gy)[R + 1],
+                      float (&gz)[R + 1], float (&gl)[R + 1]) {
 #pragma unroll
       for (int k = 0; k <= R; ++k) {
         if constexpr (PRECISE) {
-          const float4 ql = lo[v];
           gx[k] = (q.x - px[k]) + (ql.x - lx[k]);
           gy[k] = (q.y - py[k]) + (ql.y - ly[k]);
           gz[k] = (q.z - pz[k]) + (ql.z - lz[k]);
@@ -146,22 +145,38 @@ __global__ __launch_bounds__(256) void ds_tile_f32(
         gl[k] = __builtin_amdgcn_sqrtf(fmaf(gx[k], gx[k], fmaf(gy[k], gy[k], gz[k] * gz[k])));
       }
     };
+    const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    const int64_t vlast = pr.off_c + pr.n_c;  // closing vertex of the column curve
     float ax[R + 1], ay[R + 1], az[R + 1], al[R + 1], acol[R];
     float bx[R + 1], by[R + 1], bz[R + 1], bl[R + 1], bcol[R];
-    column(pr.off_c + tk.col0, ax, ay, az, al);
+    {
+      const int64_t v = pr.off_c + tk.col0;
+      column(hi[v], PRECISE ? lo[v] : zero4, ax, ay, az, al);
+    }
 #pragma unroll
     for (int k = 0; k < R; ++k)
       acol[k] = fmaf(ax[k], ax[k + 1], fmaf(ay[k], ay[k + 1], az[k] * az[k + 1]));
 
-    // One column step: the new column (j+1) from q_{j+1}, then the R segment
-    // pairs (k, j) of the quadrilaterals a=G(k,j) b=G(k+1,j) c=G(k+1,j+1)
-    // d=G(k,j+1).
+    // Column vertex j+1 and edge j of the next step, loaded one step ahead
+    // (scalar loads; the index is clamped to the closing vertex).
+    float4 nq = hi[pr.off_c + tk.col0 + 1];
+    float4 nql = PRECISE ? lo[pr.off_c + tk.col0 + 1] : zero4;
+    float4 ne = ed[pr.off_c + tk.col0];
+
+    // One column step: the new column (j+1), then the R segment pairs (k, j)
+    // of the quadrilaterals a=G(k,j) b=G(k+1,j) c=G(k+1,j+1) d=G(k,j+1).
     auto step = [&](int64_t v, const float (&gx)[R + 1], const float (&gy)[R + 1],
                     const float (&gz)[R + 1], const float (&gl)[R + 1], const float (&gcol)[R],
                     float (&nx)[R + 1], float (&ny)[R + 1], float (&nz)[R + 1],
                     float (&nl)[R + 1], float (&ncol)[R], float &accf, int &corr) {
-      const float4 e2 = ed[v];
-      column(v + 1, nx, ny, nz, nl);
+      const float4 q = nq, ql = nql, e2 = ne;
+      {
+        const int64_t w = min(v + 2, vlast);
+        nq = hi[w];
+        if constexpr (PRECISE) nql = lo[w];
+        ne = ed[min(v + 1, vlast)];
+      }
+      column(q, ql, nx, ny, nz, nl);
       float row[R + 1], diag[R];
 #pragma unroll
       for (int k = 0; k <= R; ++k) row[k] = fmaf(gx[k], nx[k], fmaf(gy[k], ny[k], gz[k] * nz[k]));
@@ -170,6 +185,8 @@ __global__ __launch_bounds__(256) void ds_tile_f32(
         ncol[k] = fmaf(nx[k], nx[k + 1], fmaf(ny[k], ny[k + 1], nz[k] * nz[k + 1]));
         diag[k] = fmaf(gx[k], nx[k + 1], fmaf(gy[k], ny[k + 1], gz[k] * nz[k + 1]));
       }
+      float xp[R], yp[R], s12[R], dt[R];
+      bool small = true;
 #pragma unroll
       for (int k = 0; k < R; ++k) {
         const float la = gl[k], lb = gl[k + 1], lc = nl[k + 1], ld = nl[k];
@@ -180,14 +197,30 @@ __global__ __launch_bounds__(256) void ds_tile_f32(
         const float cx = fmaf(e2.y, ez[k], -e2.z * ey[k]);
         const float cy = fmaf(e2.z, ex[k], -e2.x * ez[k]);
         const float cz = fmaf(e2.x, ey[k], -e2.y * ex[k]);
-        const float det = fmaf(gx[k], cx, fmaf(gy[k], cy, gz[k] * cz));
-        const float xp = fmaf(div1, div2, -det * det);
-        const float s12 = div1 + div2;
-        const float yp = det * s12;
-        const float th = atan2_fast(yp, xp);
-        const int cr = (s12 < 0.f && det != 0.f) ? (det > 0.f ? 1 : -1) : 0;
-        accf += valid[k] ? th : 0.f;
-        corr += valid[k] ? cr : 0;
+        dt[k] = fmaf(gx[k], cx, fmaf(gy[k], cy, gz[k] * cz));
+        xp[k] = fmaf(div1, div2, -dt[k] * dt[k]);
+        s12[k] = div1 + div2;
+        yp[k] = dt[k] * s12[k];
+        // small angle: x' > 0, no branch correction, |y'/x'| <= 1/8
+        small = small && (s12[k] > 0.f) && (fmaf(0.125f, xp[k], -fabsf(yp[k])) > 0.f);
+      }
+      if (__all(small)) {
+        // atan(t) = t - t^3/3 + t^5/5 - t^7/7 for |t| <= 1/8 (error < 1e-9)
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+          const float tt = yp[k] * __builtin_amdgcn_rcpf(xp[k]);
+          const float s2 = tt * tt;
+          float c = fmaf(s2, -0.14285714285714285f, 0.2f);
+          c = fmaf(s2, c, -0.3333333333333333f);
+          c = fmaf(s2, c, 1.0f);
+          accf = fmaf(tt, c, accf);
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+          accf += atan2_fast(yp[k], xp[k]);
+          corr += (s12[k] < 0.f && dt[k] != 0.f) ? (dt[k] > 0.f ? 1 : -1) : 0;
+        }
       }
     };
 
