@@ -54,6 +54,11 @@ def main():
                     help="also run learned BH vs GPU direct sum on workload 5 (2x8M)")
     ap.add_argument("--full-ds", action="store_true",
                     help="time the full workload-5 direct sum instead of a 1/16 row sample")
+    ap.add_argument("--criterion", default="beta", choices=["beta", "mlp"],
+                    help="Barnes-Hut opening criterion for --method barneshut")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="also run a device-to-device copy of MIB mebibytes (a known-size "
+                         "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
@@ -73,7 +78,18 @@ def main():
     eng = flk.Engine(0)
     log("device:", eng.device_name())
     eng.set_curves_packed(xyz, offs)
-    params = flk.default_params(method=args.method)
+    params = flk.default_params(method=args.method, criterion=args.criterion)
+    if args.criterion == "mlp":
+        from fastlinkingnumbers_amd import criterion
+        eng.mlp_set_weights(*criterion.load_weights())
+    if args.calibrate_copy:
+        import torch  # device memory plumbing for the calibration copy only
+        src = torch.empty(args.calibrate_copy << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.calibrate_copy} MiB read + {args.calibrate_copy} MiB written")
+        del src, dst
 
     def step():
         pairs = eng.potential_links()
@@ -89,7 +105,8 @@ def main():
         pairs, res, cert = step()
         times.append(time.perf_counter() - t)
         kern.append(res.stats["ms_ds_kernel"])
-        log(f"step {len(times)}: {times[-1] * 1e3:.1f} ms (direct-sum kernel {kern[-1]:.1f} ms)")
+        log(f"step {len(times)}: {times[-1] * 1e3:.1f} ms (direct-sum kernel {kern[-1]:.1f} ms, "
+            f"bh traverse {res.stats['ms_bh_traverse']:.1f} ms, visits {res.stats['bh_visits']})")
     st = res.stats
     seg_pairs = int(np.sum(sizes[pairs[:, 0]] * sizes[pairs[:, 1]])) if len(pairs) else 0
     ms_step = 1e3 * float(np.mean(times))
@@ -98,7 +115,18 @@ def main():
 
     ms_kernel = float(np.mean(kern))
     achieved = FLOP_PER_SEG_PAIR * st["ds_seg_pairs"] / (ms_kernel * 1e-3) * 1e-12 if ms_kernel > 0 else None
-    roofline = {
+    if args.method == "barneshut":
+        gbps = st["bh_visits"] * BH_BYTES_PER_VISIT / (st["ms_bh_traverse"] * 1e-3) * 1e-9
+        roofline = {
+            "bound": "hbm", "kernel": "bh_level (+bh_leaf_pairs, far field inline)",
+            "achieved": round(gbps, 1), "peak": 8000.0, "unit": "GB/s",
+            "frac": round(gbps / 8000.0, 4), "traffic": None,
+            "bytes_per_unit": BH_BYTES_PER_VISIT, "units": int(st["bh_visits"]),
+            "ms_traverse": round(st["ms_bh_traverse"], 3),
+        }
+    else:
+        roofline = None
+    roofline = roofline or {
         "bound": "valu",
         "kernel": "ds_tile_f32",
         "achieved": round(achieved, 3) if achieved else None,
@@ -112,7 +140,23 @@ def main():
     }
 
     cpu = None
-    if args.cpu_seconds > 0 and len(pairs):
+    if args.cpu_seconds > 0 and len(pairs) and args.method == "barneshut":
+        # the reference's Barnes-Hut (within-pair BFS+DFS for < 6 curves) on
+        # the whole workload when it is small enough to finish in seconds
+        if int(offs[-1]) <= 5_000_000:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import oracle as O
+            t = time.perf_counter()
+            v = [O.bh_pair(curves[i], curves[j], beta=2.0, parallel=len(curves) < 6)[0] for i, j in pairs]
+            dt = time.perf_counter() - t
+            cpu = {"value": round(seg_pairs / dt * 1e-9, 4), "unit": "Gseg-pairs/s (direct-equivalent)",
+                   "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), "kind": "port",
+                   "sample": f"whole workload through the oracle's restatement of the reference Barnes-Hut "
+                             f"(beta 2, no adaptive retry), {dt:.1f} s",
+                   "certificate_agrees": flk.certificate_from_lk(pairs, np.array(v)) == cert}
+        else:
+            log("cpu baseline skipped: workload too large for the CPU Barnes-Hut in a bounded time")
+    elif args.cpu_seconds > 0 and len(pairs):
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O  # CPU oracle: the baseline leg only
         cost = sizes[pairs[:, 0]] * sizes[pairs[:, 1]]
