@@ -47,7 +47,8 @@ class FlkParams(C.Structure):
         ("leaf_size", C.c_int32),
         ("fp64", C.c_int32),
         ("mlp_log_tol", C.c_double),
-        ("reserved", C.c_int32 * 8),
+        ("bh_batch_ksegs", C.c_int32),
+        ("reserved", C.c_int32 * 7),
     ]
 
 

@@ -25,7 +25,8 @@
 // estimate, the rest split the larger node (a leaf is never split) into the
 // next level.  Levels are processed in chunks with a LIFO stack, so memory is
 // bounded by depth x chunk whatever the frontier width.  Per-pair sums and
-// error estimates are FP64, reduced per wave before one atomic add.
+// error estimates are FP64; a block of 1024 items reduces its sums and queue
+// slots in LDS before one global atomic each.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -130,87 +131,48 @@ __device__ __forceinline__ int qidx(int i, int j) {  // symmetric 3x3 -> 6
   return i <= j ? (i == 0 ? j : (i == 1 ? 2 + j : 5)) : (j == 0 ? i : (j == 1 ? 2 + i : 5));
 }
 
-// Bottom-up moments of one level (children already done).  Leaves sum the
-// closed-form moments of their segments about the leaf box centre:
-// c += t, D += t (m - ctr)^T, Q_k += t_k (t t^T / 12 + (m-ctr)(m-ctr)^T).
-__global__ __launch_bounds__(128) void bh_moments(const BhRange *__restrict__ lev, int32_t n,
-                                                  int32_t base, const int2 *__restrict__ child,
-                                                  const uint32_t *__restrict__ order,
-                                                  const double4 *__restrict__ pts,
-                                                  Mom64 *__restrict__ m64) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  const int id = base + k;
-  const int2 ch = child[id];
-  Mom64 M;
+// Bounding box of the segments [r.lo, r.hi) of a leaf bucket.
+__device__ void leaf_box(BhRange r, const uint32_t *__restrict__ order, const double4 *__restrict__ pts,
+                         double (&box)[6]) {
   for (int t = 0; t < 3; ++t) {
-    M.box[t] = DBL_MAX;
-    M.box[3 + t] = -DBL_MAX;
-    M.c[t] = 0;
+    box[t] = DBL_MAX;
+    box[3 + t] = -DBL_MAX;
   }
-  for (int t = 0; t < 9; ++t) M.D[t] = 0;
-  for (int t = 0; t < 18; ++t) M.Q[t] = 0;
-  if (ch.x < 0) {
-    const BhRange r = lev[k];
-    for (int s = r.lo; s < r.hi; ++s) {
-      const double4 a = pts[order[s]], b = pts[order[s] + 1];
-      const double pa[3] = {a.x, a.y, a.z}, pb[3] = {b.x, b.y, b.z};
-      for (int d = 0; d < 3; ++d) {
-        M.box[d] = fmin(M.box[d], fmin(pa[d], pb[d]));
-        M.box[3 + d] = fmax(M.box[3 + d], fmax(pa[d], pb[d]));
-      }
-    }
-    double ctr[3];
-    for (int d = 0; d < 3; ++d) ctr[d] = 0.5 * (M.box[3 + d] + M.box[d]);
-    for (int s = r.lo; s < r.hi; ++s) {
-      const double4 a = pts[order[s]], b = pts[order[s] + 1];
-      const double t[3] = {b.x - a.x, b.y - a.y, b.z - a.z};
-      const double w[3] = {0.5 * (a.x + b.x) - ctr[0], 0.5 * (a.y + b.y) - ctr[1],
-                           0.5 * (a.z + b.z) - ctr[2]};
-      for (int i = 0; i < 3; ++i) {
-        M.c[i] += t[i];
-        for (int j = 0; j < 3; ++j) M.D[3 * i + j] += t[i] * w[j];
-      }
-      for (int kk = 0; kk < 3; ++kk)
-        for (int i = 0; i < 3; ++i)
-          for (int j = i; j < 3; ++j)
-            M.Q[6 * kk + qidx(i, j)] += t[kk] * ((1.0 / 12.0) * t[i] * t[j] + w[i] * w[j]);
-    }
-  } else {
-    const Mom64 &A = m64[ch.x], &B = m64[ch.y];
+  for (int s = r.lo; s < r.hi; ++s) {
+    const double4 a = pts[order[s]], b = pts[order[s] + 1];
+    const double pa[3] = {a.x, a.y, a.z}, pb[3] = {b.x, b.y, b.z};
     for (int d = 0; d < 3; ++d) {
-      M.box[d] = fmin(A.box[d], B.box[d]);
-      M.box[3 + d] = fmax(A.box[3 + d], B.box[3 + d]);
-    }
-    double ctr[3];
-    for (int d = 0; d < 3; ++d) ctr[d] = 0.5 * (M.box[3 + d] + M.box[d]);
-    for (int side = 0; side < 2; ++side) {
-      const Mom64 &C = side ? B : A;
-      double r[3];
-      for (int d = 0; d < 3; ++d) r[d] = 0.5 * (C.box[3 + d] + C.box[d]) - ctr[d];
-      for (int i = 0; i < 3; ++i) {
-        M.c[i] += C.c[i];
-        for (int j = 0; j < 3; ++j) M.D[3 * i + j] += C.D[3 * i + j] + C.c[i] * r[j];
-      }
-      for (int kk = 0; kk < 3; ++kk)
-        for (int i = 0; i < 3; ++i)
-          for (int j = i; j < 3; ++j)
-            M.Q[6 * kk + qidx(i, j)] += C.Q[6 * kk + qidx(i, j)] + r[j] * C.D[3 * kk + i] +
-                                        r[i] * C.D[3 * kk + j] + C.c[kk] * (r[i] * r[j]);
+      box[d] = fmin(box[d], fmin(pa[d], pb[d]));
+      box[3 + d] = fmax(box[3 + d], fmax(pa[d], pb[d]));
     }
   }
-  m64[id] = M;
 }
 
-// FP64 construction records -> FP32 node records.
-__global__ __launch_bounds__(256) void bh_finalize(const Mom64 *__restrict__ m64,
-                                                   const int2 *__restrict__ child,
-                                                   const BhRange *__restrict__ range, int32_t n,
-                                                   BhGeo *__restrict__ geo,
-                                                   BhMom *__restrict__ mom) {
-  const int id = blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= n) return;
-  const Mom64 &M = m64[id];
+// Adds the closed-form moments of the segments [r.lo, r.hi) about ctr:
+// c += t, D += t (m - ctr)^T, Q_k += t_k (t t^T / 12 + (m-ctr)(m-ctr)^T)
+// (reference compute_link.cpp:333-356).
+__device__ void add_segments(BhRange r, const uint32_t *__restrict__ order,
+                             const double4 *__restrict__ pts, const double (&ctr)[3], Mom64 &M) {
+  for (int s = r.lo; s < r.hi; ++s) {
+    const double4 a = pts[order[s]], b = pts[order[s] + 1];
+    const double t[3] = {b.x - a.x, b.y - a.y, b.z - a.z};
+    const double w[3] = {0.5 * (a.x + b.x) - ctr[0], 0.5 * (a.y + b.y) - ctr[1],
+                         0.5 * (a.z + b.z) - ctr[2]};
+    for (int i = 0; i < 3; ++i) {
+      M.c[i] += t[i];
+      for (int j = 0; j < 3; ++j) M.D[3 * i + j] += t[i] * w[j];
+    }
+    for (int kk = 0; kk < 3; ++kk)
+      for (int i = 0; i < 3; ++i)
+        for (int j = i; j < 3; ++j)
+          M.Q[6 * kk + qidx(i, j)] += t[kk] * ((1.0 / 12.0) * t[i] * t[j] + w[i] * w[j]);
+  }
+}
+
+// FP64 moments -> FP32 node records (centre hi/lo, radius, children, segment
+// range, moment norms for the error estimate; c, D, Q).
+__device__ void store_node(const Mom64 &M, int2 ch, BhRange r, BhGeo *__restrict__ geo,
+                           BhMom *__restrict__ mom, int id) {
   BhGeo g;
   double ctr[3], dsq = 0;
   for (int d = 0; d < 3; ++d) {
@@ -223,10 +185,10 @@ __global__ __launch_bounds__(256) void bh_finalize(const Mom64 *__restrict__ m64
   g.lx = (float)(ctr[0] - hx); g.ly = (float)(ctr[1] - hy); g.lz = (float)(ctr[2] - hz);
   g.rsq = (float)(0.25 * dsq);
   g.radius = (float)sqrt(0.25 * dsq);
-  g.c0 = child[id].x;
-  g.c1 = child[id].y;
-  g.seg0 = range[id].lo;
-  g.nseg = range[id].hi - range[id].lo;
+  g.c0 = ch.x;
+  g.c1 = ch.y;
+  g.seg0 = r.lo;
+  g.nseg = r.hi - r.lo;
   double dn = 0, qn = 0;
   for (int t = 0; t < 9; ++t) dn += M.D[t] * M.D[t];
   for (int kk = 0; kk < 3; ++kk)
@@ -245,6 +207,73 @@ __global__ __launch_bounds__(256) void bh_finalize(const Mom64 *__restrict__ m64
   for (int t = 0; t < 18; ++t) m.Q[t] = (float)M.Q[t];
   m.pad[0] = m.pad[1] = 0.f;
   mom[id] = m;
+}
+
+// Bottom-up moments of one level (children already done) and the node's FP32
+// records.  Leaf buckets keep no FP64 record: a parent adds a leaf child's
+// segments about its own centre directly (the closed form shifted exactly),
+// an internal child's record with the shift formulas of the reference
+// (compute_link.cpp:358-389, Eqs. 21-23).  Children are read one at a time.
+__global__ __launch_bounds__(128) void bh_moments(const BhRange *__restrict__ range, int32_t n,
+                                                  int32_t base, int32_t leaf,
+                                                  const int2 *__restrict__ child,
+                                                  const uint32_t *__restrict__ order,
+                                                  const double4 *__restrict__ pts,
+                                                  Mom64 *__restrict__ m64, BhGeo *__restrict__ geo,
+                                                  BhMom *__restrict__ mom) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int id = base + k;
+  const int2 ch = child[id];
+  const BhRange rr = range[id];
+  Mom64 M;
+  for (int t = 0; t < 3; ++t) M.c[t] = 0;
+  for (int t = 0; t < 9; ++t) M.D[t] = 0;
+  for (int t = 0; t < 18; ++t) M.Q[t] = 0;
+  if (ch.x < 0) {
+    leaf_box(rr, order, pts, M.box);
+    double ctr[3];
+    for (int d = 0; d < 3; ++d) ctr[d] = 0.5 * (M.box[3 + d] + M.box[d]);
+    add_segments(rr, order, pts, ctr, M);
+  } else {
+    const BhRange rc[2] = {range[ch.x], range[ch.y]};
+    const int cid[2] = {ch.x, ch.y};
+    double cb[2][6];
+    for (int side = 0; side < 2; ++side) {
+      if (rc[side].hi - rc[side].lo <= leaf) leaf_box(rc[side], order, pts, cb[side]);
+      else
+        for (int t = 0; t < 6; ++t) cb[side][t] = m64[cid[side]].box[t];
+    }
+    for (int d = 0; d < 3; ++d) {
+      M.box[d] = fmin(cb[0][d], cb[1][d]);
+      M.box[3 + d] = fmax(cb[0][3 + d], cb[1][3 + d]);
+    }
+    double ctr[3];
+    for (int d = 0; d < 3; ++d) ctr[d] = 0.5 * (M.box[3 + d] + M.box[d]);
+    for (int side = 0; side < 2; ++side) {
+      if (rc[side].hi - rc[side].lo <= leaf) {
+        add_segments(rc[side], order, pts, ctr, M);
+        continue;
+      }
+      const Mom64 &C = m64[cid[side]];
+      double r[3];
+      for (int d = 0; d < 3; ++d) r[d] = 0.5 * (cb[side][3 + d] + cb[side][d]) - ctr[d];
+      double cc[3];
+      for (int i = 0; i < 3; ++i) cc[i] = C.c[i];
+      for (int kk = 0; kk < 3; ++kk) {
+        double dk[3];
+        for (int j = 0; j < 3; ++j) dk[j] = C.D[3 * kk + j];
+        for (int i = 0; i < 3; ++i)
+          for (int j = i; j < 3; ++j)
+            M.Q[6 * kk + qidx(i, j)] += C.Q[6 * kk + qidx(i, j)] + r[j] * dk[i] + r[i] * dk[j] +
+                                        cc[kk] * (r[i] * r[j]);
+        for (int j = 0; j < 3; ++j) M.D[3 * kk + j] += dk[j] + cc[kk] * r[j];
+        M.c[kk] += cc[kk];
+      }
+    }
+    m64[id] = M;
+  }
+  store_node(M, ch, rr, geo, mom, id);
 }
 
 // ---------------------------------------------------------------- far field
@@ -378,95 +407,212 @@ __device__ __forceinline__ int64_t wave_append(unsigned long long *counter, int 
   return (int64_t)base + incl - cnt;
 }
 
-// One traversal level over a chunk of items.
-__global__ __launch_bounds__(256) void bh_level(
+// Traversal blocks: 256 threads, each classifying kItemsPerThread items, so
+// one block owns 1024 consecutive items.  Queue slots, sums and statistics are
+// reduced in LDS and published with one global atomic per block: atomics on a
+// single address serialise across the chip (one per wave capped a 4M-item
+// level at ~7 ns per atomic), and one per 1024 items removes that ceiling.
+constexpr int kTravBlock = 256;
+constexpr int kItemsPerThread = 4;
+constexpr int kTravItems = kTravBlock * kItemsPerThread;
+
+struct TravShared {
+  double v[4], e[4];
+  int pair[4];
+  int scan[4];
+  int nsplit, nleaf;
+  unsigned long long base[2];
+};
+
+// Adds every thread's (pair, v, e) into sum/err.  When all contributions of
+// the block share a pair (items of one pair cluster in the queues) this is one
+// atomic per quantity per block; mixed waves fall back to one per item.
+// Called by every thread of the block.
+__device__ void block_pair_add(int pair, bool active, double v, double e, double *__restrict__ sum,
+                               double *__restrict__ err, TravShared &sh) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const unsigned long long act = __ballot(active);
+  int p0 = -1;
+  bool uniform = true;
+  if (act) {
+    p0 = __shfl(pair, __ffsll((long long)act) - 1, 64);
+    uniform = __ballot(active && pair != p0) == 0;
+  }
+  if (!uniform && active) {
+    atomicAdd(sum + pair, v);
+    if (e != 0.0) atomicAdd(err + pair, e);
+  }
+  double a = (active && uniform) ? v : 0.0, b = (active && uniform) ? e : 0.0;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_xor(a, off, 64);
+    b += __shfl_xor(b, off, 64);
+  }
+  __syncthreads();
+  if (lane == 0) {
+    sh.pair[w] = (act && uniform) ? p0 : -1;
+    sh.v[w] = a;
+    sh.e[w] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int p = -1;
+    bool same = true;
+    for (int k = 0; k < kTravBlock / 64; ++k)
+      if (sh.pair[k] >= 0) {
+        if (p < 0) p = sh.pair[k];
+        else if (sh.pair[k] != p) same = false;
+      }
+    if (p >= 0 && same) {
+      double sa = 0.0, sb = 0.0;
+      for (int k = 0; k < kTravBlock / 64; ++k)
+        if (sh.pair[k] >= 0) {
+          sa += sh.v[k];
+          sb += sh.e[k];
+        }
+      atomicAdd(sum + p, sa);
+      if (sb != 0.0) atomicAdd(err + p, sb);
+    } else if (p >= 0) {
+      for (int k = 0; k < kTravBlock / 64; ++k)
+        if (sh.pair[k] >= 0) {
+          atomicAdd(sum + sh.pair[k], sh.v[k]);
+          if (sh.e[k] != 0.0) atomicAdd(err + sh.pair[k], sh.e[k]);
+        }
+    }
+  }
+}
+
+// Slot of this lane among the wave's lanes with `pred`, and the wave's
+// first slot from an LDS counter (one LDS atomic per wave).
+__device__ __forceinline__ int wave_lds_slot(bool pred, int *counter) {
+  const unsigned long long m = __ballot(pred);
+  if (m == 0) return -1;
+  const int lead = __ffsll((long long)m) - 1;
+  int base = 0;
+  if ((int)(threadIdx.x & 63) == lead) base = atomicAdd(counter, __popcll(m));
+  base = __shfl(base, lead, 64);
+  const unsigned long long below = m & ((1ull << (threadIdx.x & 63)) - 1ull);
+  return pred ? base + __popcll(below) : -1;
+}
+
+// One traversal level over a chunk of items.  Split items are staged in LDS
+// as (pair with the split side in its sign, unsplit node, child 0, child 1)
+// from the front of the block's stage, leaf-bucket items from the back; after
+// the block has classified its 1024 items the stage is copied out coalesced
+// behind one queue atomic each.
+__global__ __launch_bounds__(kTravBlock) void bh_level(
     const BhItem *__restrict__ items, int64_t n, const BhGeo *__restrict__ geo,
     const BhMom *__restrict__ mom, const float *__restrict__ beta2, float eps,
     const float *__restrict__ pred, float log_tol,
     BhItem *__restrict__ next, unsigned long long *__restrict__ next_count, int64_t next_cap,
     BhItem *__restrict__ leaves, unsigned long long *__restrict__ leaf_count, int64_t leaf_cap,
-    double *__restrict__ psum, double *__restrict__ perr,
-    unsigned long long *__restrict__ counters) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool live = t < n;
-  BhItem it = live ? items[t] : BhItem{0, 0, 0, 0};
-  int kind = -1;  // 0 leaf pair, 1 far, 2 split
-  float val = 0.f, e = 0.f;
-  int ca = 0, cb = 0;
-  bool split_a = false;
-  if (live) {
-    const BhGeo A = geo[it.a], B = geo[it.b];
-    const F3 r = f3((B.cx - A.cx) + (B.lx - A.lx), (B.cy - A.cy) + (B.ly - A.ly),
-                    (B.cz - A.cz) + (B.lz - A.lz));
-    const float dsq = dot3(r, r) + eps;
-    const bool la = A.c0 < 0, lb = B.c0 < 0;
-    const float rs = A.radius + B.radius;
-    // Learned criterion: far field when the spheres are disjoint and the
-    // predicted log10 error is below the tolerance; the squared predicted
-    // error is accumulated (the pair's estimate is the root of the sum, as for
-    // independent signs).  Otherwise the reference's beta test.
-    const bool far = pred ? (dsq > rs * rs + eps && pred[t] < log_tol)
-                          : (dsq > rs * rs * beta2[it.pair] + eps);
-    if (la && lb) {
-      kind = 0;
-    } else if (far) {
-      kind = 1;
-      val = far_field(mom[it.a], mom[it.b], A, B, r, dsq, e);
-      if (pred) e = exp10f(2.0f * pred[t]);
-    } else {
-      kind = 2;
-      split_a = (!la && !lb) ? !(B.rsq > A.rsq) : lb;
-      if (split_a) {
-        ca = A.c0;
-        cb = A.c1;
+    double *__restrict__ psum, double *__restrict__ perr) {
+  __shared__ int4 stage[kTravItems];
+  __shared__ TravShared sh;
+  if (threadIdx.x == 0) sh.nsplit = sh.nleaf = 0;
+  __syncthreads();
+  const int64_t b0 = (int64_t)blockIdx.x * kTravItems + threadIdx.x;
+  int tp = -1;  // this thread's far-field pair and its running sums
+  double tv = 0.0, te = 0.0;
+#pragma unroll 1
+  for (int k = 0; k < kItemsPerThread; ++k) {
+    const int64_t t = b0 + (int64_t)k * kTravBlock;
+    const bool live = t < n;
+    BhItem it = live ? items[t] : BhItem{0, 0, 0, 0};
+    int kind = -1;  // 0 leaf pair, 1 far, 2 split
+    int ca = 0, cb = 0;
+    bool split_a = false;
+    if (live) {
+      const BhGeo A = geo[it.a], B = geo[it.b];
+      const F3 r = f3((B.cx - A.cx) + (B.lx - A.lx), (B.cy - A.cy) + (B.ly - A.ly),
+                      (B.cz - A.cz) + (B.lz - A.lz));
+      const float dsq = dot3(r, r) + eps;
+      const bool la = A.c0 < 0, lb = B.c0 < 0;
+      const float rs = A.radius + B.radius;
+      // Learned criterion: far field when the spheres are disjoint and the
+      // predicted log10 error is below the tolerance; the squared predicted
+      // error is accumulated (the pair's estimate is the root of the sum, as
+      // for independent signs).  Otherwise the reference's beta test.
+      const bool far = pred ? (dsq > rs * rs + eps && pred[t] < log_tol)
+                            : (dsq > rs * rs * beta2[it.pair] + eps);
+      if (la && lb) {
+        kind = 0;
+      } else if (far) {
+        kind = 1;
+        float e = 0.f;
+        const float val = far_field(mom[it.a], mom[it.b], A, B, r, dsq, e);
+        if (pred) e = exp10f(2.0f * pred[t]);
+        if (tp < 0 || tp == it.pair) {
+          tp = it.pair;
+          tv += (double)val;
+          te += (double)e;
+        } else {
+          atomicAdd(psum + it.pair, (double)val);
+          if (e != 0.f) atomicAdd(perr + it.pair, (double)e);
+        }
       } else {
-        ca = B.c0;
-        cb = B.c1;
+        kind = 2;
+        split_a = (!la && !lb) ? !(B.rsq > A.rsq) : lb;
+        ca = split_a ? A.c0 : B.c0;
+        cb = split_a ? A.c1 : B.c1;
+      }
+    }
+    const int ss = wave_lds_slot(kind == 2, &sh.nsplit);
+    if (ss >= 0) stage[ss] = make_int4(split_a ? ~it.pair : it.pair, split_a ? it.b : it.a, ca, cb);
+    const int sl = wave_lds_slot(kind == 0, &sh.nleaf);
+    if (sl >= 0) stage[kTravItems - 1 - sl] = make_int4(it.pair, it.a, it.b, 0);
+  }
+  block_pair_add(tp, tp >= 0, tv, te, psum, perr, sh);
+  // block_pair_add ends with every stage write visible (it synchronises)
+  const int ns = sh.nsplit, nl = sh.nleaf;
+  if (threadIdx.x == 0) {
+    sh.base[0] = ns ? atomicAdd(next_count, 2ull * (unsigned long long)ns) : 0ull;
+    sh.base[1] = nl ? atomicAdd(leaf_count, (unsigned long long)nl) : 0ull;
+  }
+  __syncthreads();
+  const int64_t bn = (int64_t)sh.base[0], bl = (int64_t)sh.base[1];
+  for (int i = threadIdx.x; i < ns; i += kTravBlock) {
+    const int4 r = stage[i];
+    const int64_t s = bn + 2 * (int64_t)i;
+    if (s + 1 < next_cap) {
+      if (r.x < 0) {  // node 1 split: (child, unsplit node 2)
+        next[s] = BhItem{~r.x, r.z, r.y, 0};
+        next[s + 1] = BhItem{~r.x, r.w, r.y, 0};
+      } else {
+        next[s] = BhItem{r.x, r.y, r.z, 0};
+        next[s + 1] = BhItem{r.x, r.y, r.w, 0};
       }
     }
   }
-  // far-field contributions
-  pair_add(it.pair, kind == 1, (double)val, (double)e, psum, perr);
-  // split: two children into the next level
-  {
-    const int64_t s = wave_append(next_count, kind == 2 ? 2 : 0);
-    if (kind == 2 && s + 1 < next_cap) {
-      if (split_a) {
-        next[s] = BhItem{it.pair, ca, it.b, 0};
-        next[s + 1] = BhItem{it.pair, cb, it.b, 0};
-      } else {
-        next[s] = BhItem{it.pair, it.a, ca, 0};
-        next[s + 1] = BhItem{it.pair, it.a, cb, 0};
-      }
+  for (int i = threadIdx.x; i < nl; i += kTravBlock) {
+    const int64_t s = bl + i;
+    if (s < leaf_cap) {
+      const int4 r = stage[kTravItems - 1 - i];
+      leaves[s] = BhItem{r.x, r.y, r.z, 0};
     }
-  }
-  {
-    const int64_t s = wave_append(leaf_count, kind == 0 ? 1 : 0);
-    if (kind == 0 && s < leaf_cap) leaves[s] = it;
-  }
-  // statistics: visits, far evaluations
-  const unsigned long long far_mask = __ballot(kind == 1);
-  const unsigned long long live_mask = __ballot(live);
-  if ((threadIdx.x & 63) == 0) {
-    atomicAdd(counters + 0, (unsigned long long)__popcll(live_mask));
-    atomicAdd(counters + 1, (unsigned long long)__popcll(far_mask));
   }
 }
 
 // Exact evaluation of leaf-bucket pairs: every segment pair of the two
-// buckets with the reference leaf term, from FP32 hi/lo vertices.
-__global__ __launch_bounds__(256) void bh_leaf_pairs(
+// buckets with the reference leaf term, from FP32 hi/lo vertices.  Blocks of
+// 1024 items as in bh_level; counters[2] gets the segment-pair count.
+__global__ __launch_bounds__(kTravBlock) void bh_leaf_pairs(
     const BhItem *__restrict__ items, int64_t n, const BhGeo *__restrict__ geo,
     const uint32_t *__restrict__ order, const float4 *__restrict__ hi,
     const float4 *__restrict__ lo, const float4 *__restrict__ ed, double *__restrict__ psum,
     double *__restrict__ perr, unsigned long long *__restrict__ counters) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool live = t < n;
-  float acc = 0.f;
+  __shared__ TravShared sh;
+  const int64_t b0 = (int64_t)blockIdx.x * kTravItems + threadIdx.x;
+  int tp = -1;
+  double tv = 0.0;
   int npairs = 0;
-  BhItem it = live ? items[t] : BhItem{0, 0, 0, 0};
-  if (live) {
+#pragma unroll 1
+  for (int k = 0; k < kItemsPerThread; ++k) {
+    const int64_t t = b0 + (int64_t)k * kTravBlock;
+    if (t >= n) break;
+    const BhItem it = items[t];
     const BhGeo A = geo[it.a], B = geo[it.b];
+    float acc = 0.f;
     for (int i = A.seg0; i < A.seg0 + A.nseg; ++i) {
       const uint32_t va = order[i];
       const float4 pa = hi[va], pb = hi[va + 1], qa = lo[va], qb = lo[va + 1], ea = ed[va];
@@ -481,28 +627,24 @@ __global__ __launch_bounds__(256) void bh_leaf_pairs(
         ++npairs;
       }
     }
+    if (tp < 0 || tp == it.pair) {
+      tp = it.pair;
+      tv += (double)acc;
+    } else {
+      atomicAdd(psum + it.pair, (double)acc);
+    }
   }
-  pair_add(it.pair, live, (double)acc, 0.0, psum, perr);
+  block_pair_add(tp, tp >= 0, tv, 0.0, psum, perr, sh);
   int tot = npairs;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off, 64);
-  if ((threadIdx.x & 63) == 0 && tot) atomicAdd(counters + 2, (unsigned long long)tot);
-}
-
-// Features of every item of a chunk for the learned criterion.
-__global__ __launch_bounds__(256) void bh_features(const BhItem *__restrict__ items, int64_t n,
-                                                   const BhGeo *__restrict__ geo, float eps,
-                                                   float *__restrict__ feat) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const BhItem it = items[t];
-  const BhGeo A = geo[it.a], B = geo[it.b];
-  const float rx = (B.cx - A.cx) + (B.lx - A.lx), ry = (B.cy - A.cy) + (B.ly - A.ly),
-              rz = (B.cz - A.cz) + (B.lz - A.lz);
-  float f[FLK_MLP_NFEAT];
-  mlp_features(A, B, fmaf(rx, rx, fmaf(ry, ry, rz * rz)) + eps, f);
-#pragma unroll
-  for (int k = 0; k < FLK_MLP_NFEAT; ++k) feat[t * FLK_MLP_NFEAT + k] = f[k];
+  if ((threadIdx.x & 63) == 0) sh.scan[threadIdx.x >> 6] = tot;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long s = 0;
+    for (int k = 0; k < kTravBlock / 64; ++k) s += (unsigned long long)sh.scan[k];
+    if (s) atomicAdd(counters + 2, s);
+  }
 }
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
@@ -625,10 +767,12 @@ __global__ void bh_roots(const int32_t *__restrict__ ra, const int32_t *__restri
 // ---------------------------------------------------------------- host side
 namespace {
 
+// Node records of the trees of one call; the buffers are the context's.
 struct Forest {
   int32_t nnodes = 0;
   std::vector<int32_t> root;  // per curve (-1 if no tree)
-  DevBuf geo, mom, order;
+  DevBuf &geo, &mom, &order;
+  explicit Forest(BhScratch &b) : geo(b.geo), mom(b.mom), order(b.order) {}
 };
 
 void build_forest(flk_ctx *ctx, const std::vector<int32_t> &curves_in, int leaf, Forest &F) {
@@ -653,7 +797,9 @@ void build_forest(flk_ctx *ctx, const std::vector<int32_t> &curves_in, int leaf,
   }
   const int64_t S = seg_off[nc];
   if (S >= INT32_MAX / 2) throw Error(FLK_ERR_ARG, "too many Barnes-Hut segments");
-  DevBuf d_seg, d_vtx, d_box, d_k0, d_k1, d_v0, d_tmp;
+  BhScratch &B = ctx->bh;
+  DevBuf &d_seg = B.seg, &d_vtx = B.vtx, &d_box = B.box, &d_k0 = B.keys0, &d_k1 = B.keys1, &d_v0 = B.ids0,
+         &d_tmp = B.tmp;
   d_seg.reserve(sizeof(int64_t) * (nc + 1));
   d_vtx.reserve(sizeof(int64_t) * nc);
   d_box.reserve(sizeof(double) * 6 * nc);
@@ -678,16 +824,13 @@ void build_forest(flk_ctx *ctx, const std::vector<int32_t> &curves_in, int leaf,
   FLK_HIP(hipcub::DeviceRadixSort::SortPairs(d_tmp.ptr, tb, d_k0.as<uint64_t>(), d_k1.as<uint64_t>(),
                                              d_v0.as<uint32_t>(), F.order.as<uint32_t>(), (int)S, 0,
                                              end_bit, s));
-  d_k0.release();
-  d_k1.release();
-  d_v0.release();
 
   // level-by-level node creation
   std::vector<int32_t> level_base{0}, level_n{nc};
   std::vector<BhRange> roots(nc);
   for (int32_t k = 0; k < nc; ++k) roots[k] = BhRange{(int32_t)seg_off[k], (int32_t)seg_off[k + 1]};
   const int64_t max_nodes = 2 * S + nc;
-  DevBuf d_range, d_child, d_flag, d_scan;
+  DevBuf &d_range = B.range, &d_child = B.child, &d_flag = B.flag, &d_scan = B.scan;
   d_range.reserve(sizeof(BhRange) * max_nodes);
   d_child.reserve(sizeof(int2) * max_nodes);
   d_flag.reserve(sizeof(int32_t) * (S + nc + 1));
@@ -720,114 +863,134 @@ void build_forest(flk_ctx *ctx, const std::vector<int32_t> &curves_in, int leaf,
     total += 2 * nsplit;
   }
   F.nnodes = total;
-  DevBuf d_m64;
+  DevBuf &d_m64 = B.m64;
   d_m64.reserve(sizeof(Mom64) * total);
-  for (int lev = (int)level_base.size() - 1; lev >= 0; --lev) {
-    const int32_t n = level_n[lev];
-    hipLaunchKernelGGL(bh_moments, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s,
-                       d_range.as<BhRange>() + level_base[lev], n, level_base[lev],
-                       d_child.as<int2>(), F.order.as<uint32_t>(), cv.dpts.as<double4>(),
-                       d_m64.as<Mom64>());
-    FLK_HIP(hipGetLastError());
-  }
   F.geo.reserve(sizeof(BhGeo) * total);
   F.mom.reserve(sizeof(BhMom) * total);
-  hipLaunchKernelGGL(bh_finalize, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
-                     d_m64.as<Mom64>(), d_child.as<int2>(), d_range.as<BhRange>(), total,
-                     F.geo.as<BhGeo>(), F.mom.as<BhMom>());
-  FLK_HIP(hipGetLastError());
+  for (int lev = (int)level_base.size() - 1; lev >= 0; --lev) {
+    const int32_t n = level_n[lev];
+    hipLaunchKernelGGL(bh_moments, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, d_range.as<BhRange>(),
+                       n, level_base[lev], leaf, d_child.as<int2>(), F.order.as<uint32_t>(),
+                       cv.dpts.as<double4>(), d_m64.as<Mom64>(), F.geo.as<BhGeo>(), F.mom.as<BhMom>());
+    FLK_HIP(hipGetLastError());
+  }
   FLK_HIP(hipStreamSynchronize(s));
   for (int32_t k = 0; k < nc; ++k) F.root[cs[k]] = k;  // roots are nodes 0..nc-1
-  d_m64.release();
-  d_range.release();
-  d_child.release();
 }
 
 // Traverses the given pairs (indices into the forest roots) with per-pair
 // beta; adds into psum/perr (device, per pair).
+//
+// Items live in level buffers processed in chunks of up to kChunk items; a
+// level that fits one chunk is consumed whole and its buffer swapped with the
+// next level's (breadth-first, two buffers), a wider one pushes the next
+// level on a LIFO stack so memory stays bounded by depth x chunk.  The host
+// reads two queue counts per launch (pinned memory); the device counters
+// rotate through a pre-zeroed array instead of a clear per launch.
 void traverse(flk_ctx *ctx, Forest &F, const std::vector<int32_t> &ra, const std::vector<int32_t> &rb,
-              const std::vector<float> &beta2, float eps, DevBuf &d_sum, DevBuf &d_err,
+              const std::vector<float> &beta2, float eps, double *d_sum, double *d_err,
               unsigned long long stats[3], bool use_mlp = false, float log_tol = 0.f) {
   hipStream_t s = ctx->stream;
+  BhScratch &B = ctx->bh;
   const int64_t np = (int64_t)ra.size();
   if (np == 0) return;
-  DevBuf d_ra, d_rb, d_beta, d_cnt;
-  d_ra.reserve(sizeof(int32_t) * np);
-  d_rb.reserve(sizeof(int32_t) * np);
-  d_beta.reserve(sizeof(float) * np);
-  d_cnt.reserve(sizeof(unsigned long long) * 8);
-  FLK_HIP(hipMemcpyAsync(d_ra.ptr, ra.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice, s));
-  FLK_HIP(hipMemcpyAsync(d_rb.ptr, rb.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice, s));
-  FLK_HIP(hipMemcpyAsync(d_beta.ptr, beta2.data(), sizeof(float) * np, hipMemcpyHostToDevice, s));
-  FLK_HIP(hipMemsetAsync(d_cnt.ptr, 0, sizeof(unsigned long long) * 8, s));
-  unsigned long long *cnt = d_cnt.as<unsigned long long>();  // 0 visits 1 far 2 leaf pairs 3 next 4 leaves
+  constexpr int64_t kChunk = 1 << 24;
+  constexpr int kSlots = 1024;  // count pairs per zeroing of the counter array
+  B.roots_a.reserve(sizeof(int32_t) * np);
+  B.roots_b.reserve(sizeof(int32_t) * np);
+  B.beta2.reserve(sizeof(float) * np);
+  B.cnt.reserve(sizeof(unsigned long long) * (4 + 2 * kSlots));
+  if (!B.host_cnt) FLK_HIP(hipHostMalloc((void **)&B.host_cnt, 2 * sizeof(unsigned long long)));
+  FLK_HIP(hipMemcpyAsync(B.roots_a.ptr, ra.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice, s));
+  FLK_HIP(hipMemcpyAsync(B.roots_b.ptr, rb.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice, s));
+  FLK_HIP(hipMemcpyAsync(B.beta2.ptr, beta2.data(), sizeof(float) * np, hipMemcpyHostToDevice, s));
+  unsigned long long *cnt = B.cnt.as<unsigned long long>();  // [2] leaf segment pairs, [4 + 2k] queues
+  FLK_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * (4 + 2 * kSlots), s));
 
-  // LIFO stack of level buffers: each holds items still to process.
-  const int64_t chunk = 1 << 22;
   struct Level {
-    DevBuf buf;
-    int64_t n = 0;
+    DevBuf *buf;
+    int64_t n;
   };
-  std::vector<Level> stack(1);
-  stack[0].buf.reserve(sizeof(BhItem) * std::max<int64_t>(np, 1));
-  stack[0].n = np;
+  std::vector<Level> stack;
+  size_t used = 0;  // level buffers handed out from B.levels
+  auto level_buf = [&](int64_t items) -> DevBuf * {
+    if (B.levels.size() <= used) B.levels.push_back(std::make_unique<DevBuf>());
+    DevBuf *b = B.levels[used++].get();
+    b->reserve(sizeof(BhItem) * std::max<int64_t>(items, 1));
+    return b;
+  };
+  stack.push_back(Level{level_buf(np), np});
   hipLaunchKernelGGL(bh_roots, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s,
-                     d_ra.as<int32_t>(), d_rb.as<int32_t>(), np, stack[0].buf.as<BhItem>());
+                     B.roots_a.as<int32_t>(), B.roots_b.as<int32_t>(), np, stack[0].buf->as<BhItem>());
   FLK_HIP(hipGetLastError());
-  DevBuf leaves, feat, pred;
-  leaves.reserve(sizeof(BhItem) * chunk * 2);
+  DevBuf *spare = level_buf(2 * std::min(np, kChunk));
   if (use_mlp) {
     mlp_upload(ctx);
-    feat.reserve(sizeof(float) * FLK_MLP_NFEAT * chunk);
-    pred.reserve(sizeof(float) * chunk);
+    B.pred.reserve(sizeof(float) * std::min(np, kChunk));
   }
-  std::vector<Level> pool;  // reusable buffers
-  size_t depth = 1;
-  while (depth > 0) {
-    Level &top = stack[depth - 1];
+  unsigned long long visits = 0, far = 0;
+  int64_t iter = 0;
+  while (!stack.empty()) {
+    Level &top = stack.back();
     if (top.n == 0) {
-      --depth;
+      spare = top.buf;  // an emptied level's buffer is reused
+      stack.pop_back();
       continue;
     }
-    const int64_t take = std::min<int64_t>(top.n, chunk);
+    const int64_t take = std::min<int64_t>(top.n, kChunk);
     top.n -= take;
-    const BhItem *src = top.buf.as<BhItem>() + top.n;
-    if (stack.size() <= depth) stack.resize(depth + 1);
-    Level &nx = stack[depth];
-    nx.buf.reserve(sizeof(BhItem) * 2 * chunk);
-    FLK_HIP(hipMemsetAsync(cnt + 3, 0, 2 * sizeof(unsigned long long), s));
-    if (use_mlp) {
-      hipLaunchKernelGGL(bh_features, dim3((unsigned)((take + 255) / 256)), dim3(256), 0, s, src, take,
-                         F.geo.as<BhGeo>(), eps, feat.as<float>());
-      FLK_HIP(hipGetLastError());
-      mlp_forward_device(ctx, feat.as<float>(), take, pred.as<float>());
+    const BhItem *src = top.buf->as<BhItem>() + top.n;
+    spare->reserve(sizeof(BhItem) * 2 * take);
+    if (B.leaves.bytes < sizeof(BhItem) * take) {
+      FLK_HIP(hipStreamSynchronize(s));  // the previous leaf launch may still read it
+      B.leaves.reserve(sizeof(BhItem) * take);
     }
-    hipLaunchKernelGGL(bh_level, dim3((unsigned)((take + 255) / 256)), dim3(256), 0, s, src, take,
-                       F.geo.as<BhGeo>(), F.mom.as<BhMom>(), d_beta.as<float>(), eps,
-                       use_mlp ? pred.as<float>() : nullptr, log_tol,
-                       nx.buf.as<BhItem>(), cnt + 3, 2 * chunk, leaves.as<BhItem>(), cnt + 4,
-                       2 * chunk, d_sum.as<double>(), d_err.as<double>(), cnt);
+    if (use_mlp) {
+      B.pred.reserve(sizeof(float) * take);
+      mlp_predict_items(ctx, src, take, F.geo.as<BhGeo>(), eps, B.pred.as<float>());
+    }
+    const int slot = (int)(iter % kSlots);
+    if (slot == 0 && iter > 0)
+      FLK_HIP(hipMemsetAsync(cnt + 4, 0, sizeof(unsigned long long) * 2 * kSlots, s));
+    unsigned long long *qc = cnt + 4 + 2 * slot;  // [0] next items, [1] leaf items
+    hipLaunchKernelGGL(bh_level, dim3((unsigned)((take + kTravItems - 1) / kTravItems)), dim3(kTravBlock),
+                       0, s, src, take, F.geo.as<BhGeo>(), F.mom.as<BhMom>(), B.beta2.as<float>(), eps,
+                       use_mlp ? B.pred.as<float>() : nullptr, log_tol, spare->as<BhItem>(), qc,
+                       2 * take, B.leaves.as<BhItem>(), qc + 1, take, d_sum, d_err);
     FLK_HIP(hipGetLastError());
-    unsigned long long nn[2] = {0, 0};
-    FLK_HIP(hipMemcpyAsync(nn, cnt + 3, sizeof nn, hipMemcpyDeviceToHost, s));
+    FLK_HIP(hipMemcpyAsync(B.host_cnt, qc, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     FLK_HIP(hipStreamSynchronize(s));
-    if ((int64_t)nn[0] > 2 * chunk || (int64_t)nn[1] > 2 * chunk)
+    ++iter;
+    const unsigned long long nn[2] = {B.host_cnt[0], B.host_cnt[1]};
+    if ((int64_t)nn[0] > 2 * take || (int64_t)nn[1] > take)
       throw Error(FLK_ERR_STATE, "Barnes-Hut level buffer overflow");
+    // every item is exactly one of: leaf pair, far field, split (two children)
+    visits += (unsigned long long)take;
+    far += (unsigned long long)take - nn[0] / 2 - nn[1];
     if (nn[1] > 0) {
       const Curves &cv = ctx->curves;
-      hipLaunchKernelGGL(bh_leaf_pairs, dim3((unsigned)((nn[1] + 255) / 256)), dim3(256), 0, s,
-                         leaves.as<BhItem>(), (int64_t)nn[1], F.geo.as<BhGeo>(),
+      hipLaunchKernelGGL(bh_leaf_pairs, dim3((unsigned)((nn[1] + kTravItems - 1) / kTravItems)),
+                         dim3(kTravBlock), 0, s, B.leaves.as<BhItem>(), (int64_t)nn[1], F.geo.as<BhGeo>(),
                          F.order.as<uint32_t>(), cv.hi.as<float4>(), cv.lo.as<float4>(),
-                         cv.ed.as<float4>(), d_sum.as<double>(), d_err.as<double>(), cnt);
+                         cv.ed.as<float4>(), d_sum, d_err, cnt);
       FLK_HIP(hipGetLastError());
     }
-    nx.n = (int64_t)nn[0];
-    if (nx.n > 0) ++depth;
+    if (nn[0] == 0) continue;
+    if (top.n == 0) {
+      // the level is consumed: its children replace it
+      std::swap(top.buf, spare);
+      top.n = (int64_t)nn[0];
+    } else {
+      stack.push_back(Level{spare, (int64_t)nn[0]});
+      spare = level_buf(2 * kChunk);
+    }
   }
   unsigned long long st[3];
   FLK_HIP(hipMemcpyAsync(st, cnt, sizeof st, hipMemcpyDeviceToHost, s));
   FLK_HIP(hipStreamSynchronize(s));
-  for (int k = 0; k < 3; ++k) stats[k] += st[k];
+  stats[0] += visits;
+  stats[1] += far;
+  stats[2] += st[2];
 }
 
 }  // namespace
@@ -838,7 +1001,7 @@ int64_t mlp_samples(flk_ctx *ctx, int32_t a, int32_t b, int64_t cap, uint64_t se
   hipStream_t s = ctx->stream;
   if (cap <= 0) return 0;
   if (cv.n[a] < 2 || cv.n[b] < 2) return 0;
-  Forest F;
+  Forest F(ctx->bh);
   std::vector<int32_t> cs{std::min(a, b), std::max(a, b)};
   if (a == b) cs.resize(1);
   build_forest(ctx, cs, 1, F);
@@ -895,48 +1058,22 @@ int64_t mlp_samples(flk_ctx *ctx, int32_t a, int32_t b, int64_t cap, uint64_t se
   return got;
 }
 
-void bh_run(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, const flk_params &prm,
-            std::vector<double> &lk, std::vector<double> &err, std::vector<uint32_t> &flags) {
+namespace {
+
+// Trees, traversal and the adaptive-beta / learned-criterion re-run for one
+// batch of Barnes–Hut pairs (indices into `pairs`); results into lk/err/flags.
+void bh_batch(flk_ctx *ctx, const int32_t *pairs, const std::vector<int64_t> &bh_idx, const flk_params &prm,
+              std::vector<double> &lk, std::vector<double> &err, std::vector<uint32_t> &flags) {
   Curves &cv = ctx->curves;
   hipStream_t s = ctx->stream;
-  lk.assign(npairs, 0.0);
-  err.assign(npairs, 0.0);
-  flags.assign(npairs, 0u);
-  if (npairs == 0) return;
   const bool use_mlp = prm.criterion == FLK_CRIT_MLP;
-  if (use_mlp && !ctx->mlp.loaded)
-    throw Error(FLK_ERR_STATE, "the learned criterion needs weights (flk_mlp_set_weights)");
-  // Pairs below the direct threshold (or with a curve too short for a tree)
-  // go to the direct sum; the rest to Barnes–Hut.
-  std::vector<int64_t> ds_idx, bh_idx;
-  std::vector<int32_t> ds_pairs;
-  std::vector<char> used(cv.ncurves, 0);
-  for (int64_t k = 0; k < npairs; ++k) {
-    const int32_t a = pairs[2 * k], b = pairs[2 * k + 1];
-    const int64_t work = (int64_t)cv.n[a] * cv.n[b];
-    if (cv.n[a] < 2 || cv.n[b] < 2 || work < prm.bh_direct_threshold) {
-      ds_idx.push_back(k);
-      ds_pairs.push_back(a);
-      ds_pairs.push_back(b);
-      flags[k] |= FLK_PAIR_DIRECT;
-    } else {
-      bh_idx.push_back(k);
-      used[a] = used[b] = 1;
-    }
-  }
-  if (!ds_idx.empty()) {
-    std::vector<double> v;
-    float ms = 0.f;
-    ds_run(ctx, ds_pairs.data(), (int64_t)ds_idx.size(), prm.fp64 != 0, v, &ms);
-    ctx->stats.ms_ds_kernel += ms;
-    for (size_t t = 0; t < ds_idx.size(); ++t) lk[ds_idx[t]] = v[t];
-  }
-  if (bh_idx.empty()) return;
   const int leaf = std::max(1, prm.leaf_size);
+  std::vector<char> used(cv.ncurves, 0);
+  for (int64_t k : bh_idx) used[pairs[2 * k]] = used[pairs[2 * k + 1]] = 1;
   std::vector<int32_t> cs;
   for (int32_t c = 0; c < cv.ncurves; ++c)
     if (used[c]) cs.push_back(c);
-  Forest F;
+  Forest F(ctx->bh);
   auto t0 = std::chrono::steady_clock::now();
   build_forest(ctx, cs, leaf, F);
   ctx->stats.ms_bh_build +=
@@ -950,17 +1087,18 @@ void bh_run(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, const flk_params
     beta2[t] = (float)(prm.beta_init * prm.beta_init);
   }
   const float eps = (float)(prm.eps_sq * cv.scale * cv.scale);
-  DevBuf d_sum, d_err;
-  d_sum.reserve(sizeof(double) * nb);
-  d_err.reserve(sizeof(double) * nb);
-  FLK_HIP(hipMemsetAsync(d_sum.ptr, 0, sizeof(double) * nb, s));
-  FLK_HIP(hipMemsetAsync(d_err.ptr, 0, sizeof(double) * nb, s));
+  // per-pair sums: [0, nb) first pass, [nb, 2 nb) re-runs
+  ctx->bh.sum.reserve(sizeof(double) * 2 * nb);
+  ctx->bh.err.reserve(sizeof(double) * 2 * nb);
+  double *d_sum = ctx->bh.sum.as<double>(), *d_err = ctx->bh.err.as<double>();
+  FLK_HIP(hipMemsetAsync(d_sum, 0, sizeof(double) * 2 * nb, s));
+  FLK_HIP(hipMemsetAsync(d_err, 0, sizeof(double) * 2 * nb, s));
   unsigned long long st[3] = {0, 0, 0};
   t0 = std::chrono::steady_clock::now();
   traverse(ctx, F, ra, rb, beta2, eps, d_sum, d_err, st, use_mlp, (float)prm.mlp_log_tol);
   std::vector<double> v(nb), e(nb);
-  FLK_HIP(hipMemcpyAsync(v.data(), d_sum.ptr, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
-  FLK_HIP(hipMemcpyAsync(e.data(), d_err.ptr, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+  FLK_HIP(hipMemcpyAsync(v.data(), d_sum, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+  FLK_HIP(hipMemcpyAsync(e.data(), d_err, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
   FLK_HIP(hipStreamSynchronize(s));
   // Adaptive beta (reference linking_number_certificate.cpp:165-184): scale
   // the estimate, target beta = min(beta_init (E / target)^(1/4), limit).
@@ -993,15 +1131,11 @@ void bh_run(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, const flk_params
   }
   if (!redo.empty()) {
     const int64_t nr = (int64_t)redo.size();
-    DevBuf d_sum2, d_err2;
-    d_sum2.reserve(sizeof(double) * nr);
-    d_err2.reserve(sizeof(double) * nr);
-    FLK_HIP(hipMemsetAsync(d_sum2.ptr, 0, sizeof(double) * nr, s));
-    FLK_HIP(hipMemsetAsync(d_err2.ptr, 0, sizeof(double) * nr, s));
+    double *d_sum2 = d_sum + nb, *d_err2 = d_err + nb;
     traverse(ctx, F, ra2, rb2, b2, eps, d_sum2, d_err2, st, use_mlp, (float)prm.mlp_log_tol - 1.0f);
     std::vector<double> v2(nr), e2(nr);
-    FLK_HIP(hipMemcpyAsync(v2.data(), d_sum2.ptr, sizeof(double) * nr, hipMemcpyDeviceToHost, s));
-    FLK_HIP(hipMemcpyAsync(e2.data(), d_err2.ptr, sizeof(double) * nr, hipMemcpyDeviceToHost, s));
+    FLK_HIP(hipMemcpyAsync(v2.data(), d_sum2, sizeof(double) * nr, hipMemcpyDeviceToHost, s));
+    FLK_HIP(hipMemcpyAsync(e2.data(), d_err2, sizeof(double) * nr, hipMemcpyDeviceToHost, s));
     FLK_HIP(hipStreamSynchronize(s));
     for (int64_t t = 0; t < nr; ++t) {
       v[redo[t]] = v2[t];
@@ -1019,6 +1153,87 @@ void bh_run(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, const flk_params
     lk[bh_idx[t]] = v[t];
     err[bh_idx[t]] = e[t];
   }
+}
+
+// Device bytes the trees and their build records take per segment (two
+// 192-byte node records, one 288-byte FP64 record per internal node, ranges,
+// children, sort keys and scan flags), with headroom.
+constexpr double kBhBytesPerSegment = 800.0;
+
+}  // namespace
+
+void bh_run(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, const flk_params &prm,
+            std::vector<double> &lk, std::vector<double> &err, std::vector<uint32_t> &flags) {
+  Curves &cv = ctx->curves;
+  lk.assign(npairs, 0.0);
+  err.assign(npairs, 0.0);
+  flags.assign(npairs, 0u);
+  if (npairs == 0) return;
+  const bool use_mlp = prm.criterion == FLK_CRIT_MLP;
+  if (use_mlp && !ctx->mlp.loaded)
+    throw Error(FLK_ERR_STATE, "the learned criterion needs weights (flk_mlp_set_weights)");
+  // Pairs below the direct threshold (or with a curve too short for a tree)
+  // go to the direct sum; the rest to Barnes–Hut.
+  std::vector<int64_t> ds_idx, bh_idx;
+  std::vector<int32_t> ds_pairs;
+  for (int64_t k = 0; k < npairs; ++k) {
+    const int32_t a = pairs[2 * k], b = pairs[2 * k + 1];
+    const int64_t work = (int64_t)cv.n[a] * cv.n[b];
+    if (cv.n[a] < 2 || cv.n[b] < 2 || work < prm.bh_direct_threshold) {
+      ds_idx.push_back(k);
+      ds_pairs.push_back(a);
+      ds_pairs.push_back(b);
+      flags[k] |= FLK_PAIR_DIRECT;
+    } else {
+      bh_idx.push_back(k);
+    }
+  }
+  if (!ds_idx.empty()) {
+    std::vector<double> v;
+    float ms = 0.f;
+    ds_run(ctx, ds_pairs.data(), (int64_t)ds_idx.size(), prm.fp64 != 0, v, &ms);
+    ctx->stats.ms_ds_kernel += ms;
+    for (size_t t = 0; t < ds_idx.size(); ++t) lk[ds_idx[t]] = v[t];
+  }
+  if (bh_idx.empty()) return;
+  // Batches of pairs whose curves' trees fit the device: the segment budget
+  // is bh_batch_ksegs thousand, or what free memory (plus the scratch this
+  // context already holds) allows.  A curve shared by two batches is built
+  // in both.
+  int64_t budget;
+  if (prm.bh_batch_ksegs > 0) {
+    budget = (int64_t)prm.bh_batch_ksegs * 1000;
+  } else {
+    size_t free_b = 0, total_b = 0;
+    FLK_HIP(hipMemGetInfo(&free_b, &total_b));
+    budget = (int64_t)(0.6 * (double)(free_b + ctx->bh.bytes()) / kBhBytesPerSegment);
+  }
+  budget = std::max<int64_t>(budget, 1);
+  std::vector<char> in_batch(cv.ncurves, 0);
+  std::vector<int32_t> members;
+  std::vector<int64_t> batch;
+  int64_t segs = 0;
+  auto flush = [&]() {
+    if (batch.empty()) return;
+    bh_batch(ctx, pairs, batch, prm, lk, err, flags);
+    for (int32_t c : members) in_batch[c] = 0;
+    members.clear();
+    batch.clear();
+    segs = 0;
+  };
+  for (int64_t k : bh_idx) {
+    const int32_t a = pairs[2 * k], b = pairs[2 * k + 1];
+    const int64_t add = (in_batch[a] ? 0 : cv.n[a]) + (in_batch[b] || a == b ? 0 : cv.n[b]);
+    if (!batch.empty() && segs + add > budget) flush();
+    for (int32_t c : {a, b})
+      if (!in_batch[c]) {
+        in_batch[c] = 1;
+        members.push_back(c);
+        segs += cv.n[c];
+      }
+    batch.push_back(k);
+  }
+  flush();
 }
 
 }  // namespace flk

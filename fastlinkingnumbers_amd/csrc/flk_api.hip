@@ -92,6 +92,7 @@ void flk_destroy(flk_ctx *ctx) {
                          &ctx->d_boxes, &ctx->d_pls, &ctx->d_count, &ctx->d_misc,
                          &ctx->mlp.d_params, &ctx->mlp.d_norm, &ctx->mlp.d_bf16};
   for (auto *b : bufs) b->release();
+  ctx->bh.release();
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -235,6 +236,7 @@ int32_t flk_linking_numbers(flk_ctx *ctx, const int32_t *pairs, int64_t npairs,
       ctx->stats.ms_kernel = ms;
       ctx->stats.ms_ds_kernel = ms;
     } else if (prm.method == FLK_BARNESHUT) {
+      if (prm.bh_batch_ksegs < 0) throw Error(FLK_ERR_ARG, "bh_batch_ksegs must be >= 0");
       flk::bh_run(ctx, pairs, npairs, prm, v, e, f);
     } else {
       throw Error(FLK_ERR_ARG, "invalid method");

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <string>
+#include <memory>
 #include <vector>
 
 #include "flk.h"
@@ -112,6 +113,36 @@ struct Curves {
   bool inexact = false;          // some stored coordinate needs the FP32 lo part
 };
 
+// Barnes–Hut device scratch, kept by the context between calls (grow-only)
+// so repeated certificates do not pay hipMalloc/hipFree for the trees, their
+// FP64 build records and the traversal queues.
+struct BhScratch {
+  // forest
+  DevBuf geo, mom, order, m64, range, child, flag, scan, keys0, keys1, ids0, tmp, seg, vtx, box;
+  // traversal
+  DevBuf roots_a, roots_b, beta2, cnt, leaves, pred, sum, err;
+  std::vector<std::unique_ptr<DevBuf>> levels;
+  unsigned long long *host_cnt = nullptr;  // pinned, 2 words
+  template <typename Fn> void each(Fn fn) {
+    DevBuf *all[] = {&geo, &mom, &order, &m64, &range, &child, &flag, &scan, &keys0, &keys1, &ids0,
+                     &tmp, &seg, &vtx, &box, &roots_a, &roots_b, &beta2, &cnt, &leaves, &pred,
+                     &sum, &err};
+    for (DevBuf *b : all) fn(*b);
+    for (auto &b : levels) fn(*b);
+  }
+  size_t bytes() {
+    size_t n = 0;
+    each([&](DevBuf &b) { n += b.bytes; });
+    return n;
+  }
+  void release() {
+    each([](DevBuf &b) { b.release(); });
+    levels.clear();
+    if (host_cnt) (void)hipHostFree(host_cnt);
+    host_cnt = nullptr;
+  }
+};
+
 struct MlpState {
   std::vector<float> params;  // FLK_MLP_NPARAM FP32 masters
   std::vector<float> norm;    // mean[12], std[12]
@@ -132,6 +163,7 @@ struct flk_ctx {
   flk_stats stats{};
   std::vector<int32_t> pls_pairs;
   flk::MlpState mlp;
+  flk::BhScratch bh;
   // scratch
   flk::DevBuf d_pairs, d_tasks, d_cost, d_task_theta, d_task_corr, d_pair_task0,
       d_pair_out, d_pair_corr, d_boxes, d_pls, d_count, d_misc;
@@ -160,6 +192,12 @@ void mlp_forward(flk_ctx *ctx, const float *x, int64_t n, float *y);
 void mlp_upload(flk_ctx *ctx);
 // Forward pass on device buffers (features n x 12, predictions n).
 void mlp_forward_device(flk_ctx *ctx, const float *dx, int64_t n, float *dy);
+// Learned-criterion predictions for Barnes–Hut work items (log10 error, or
+// +inf where the decision does not depend on it), from the node records.
+struct BhItem;
+struct BhGeo;
+void mlp_predict_items(flk_ctx *ctx, const BhItem *items, int64_t n, const BhGeo *geo, float eps,
+                       float *pred);
 float mlp_gradient(flk_ctx *ctx, const float *x, const float *y, int64_t n, float *grad);
 float mlp_train(flk_ctx *ctx, const float *x, const float *y, int64_t n, int epochs, int batch,
                 float lr, uint64_t seed);

@@ -17,6 +17,7 @@
 #include <cstring>
 #include <random>
 
+#include "bh_types.h"
 #include "flk_internal.h"
 
 namespace flk {
@@ -30,7 +31,10 @@ constexpr int H = 64;       // hidden width
 constexpr int KP = 32;      // padded input width
 constexpr int OFF_W1 = 0, OFF_B1 = 768, OFF_W2 = 832, OFF_B2 = 4928, OFF_W3 = 4992, OFF_B3 = 5056;
 
-__device__ __forceinline__ float sigm(float z) { return 1.0f / (1.0f + __expf(-z)); }
+// Logistic function with the hardware reciprocal (1 ulp; the activations are
+// rounded to bf16 or summed into an FP32 output): a full IEEE division would
+// cost ten instructions per activation.
+__device__ __forceinline__ float sigm(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 
 struct WaveLds {          // per-wave staging, bf16 row-major tiles of 16 rows
   __bf16 x[16][KP];
@@ -149,21 +153,185 @@ __device__ void stage_x(const BlockLds &L, WaveLds &W, const float *__restrict__
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 }
 
+// Inference: both layers as C = W . X^T (neurons x samples), so lane l ends
+// layer 1 holding neurons nb*16 + 4*(l/16) + i of sample l%16 -- exactly the
+// B fragment of layer 2 once W2's reduction index is permuted to the same
+// order (k-slot 8g + j of step s <-> neuron (2s + j/4)*16 + 4g + j%4).
+// The A fragments of both layers live in VGPRs for the whole kernel, the
+// biases, output weights and input normalisation in LDS; activations never
+// leave registers.  One tile = 16 samples = 12 MFMAs per wave.
+struct InferRegs {
+  bf16x8 a1[4], a2[4][2];
+};
+struct InferLds {
+  float b1[H], b2[H], w3[H];
+  float mean[16], istd[16];  // features 12..15 pad: mean 0, istd 0
+  float b3;
+};
+
+__device__ void load_infer(const float *__restrict__ P, const float *__restrict__ norm, InferRegs &R,
+                           InferLds &L) {
+  const int l = threadIdx.x & 63, g = l >> 4, c = l & 15;
+#pragma unroll
+  for (int nb = 0; nb < 4; ++nb) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int f = 8 * g + j;
+      R.a1[nb][j] = (__bf16)(f < FLK_MLP_NFEAT ? P[OFF_W1 + (nb * 16 + c) * FLK_MLP_NFEAT + f] : 0.0f);
+    }
+#pragma unroll
+    for (int st = 0; st < 2; ++st)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int neuron = (2 * st + j / 4) * 16 + 4 * g + (j & 3);
+        R.a2[nb][st][j] = (__bf16)P[OFF_W2 + (nb * 16 + c) * H + neuron];
+      }
+  }
+  for (int t = threadIdx.x; t < H; t += blockDim.x) {
+    L.b1[t] = P[OFF_B1 + t];
+    L.b2[t] = P[OFF_B2 + t];
+    L.w3[t] = P[OFF_W3 + t];
+  }
+  for (int t = threadIdx.x; t < 16; t += blockDim.x) {
+    L.mean[t] = t < FLK_MLP_NFEAT ? norm[t] : 0.f;
+    L.istd[t] = t < FLK_MLP_NFEAT ? 1.0f / norm[FLK_MLP_NFEAT + t] : 0.f;
+  }
+  if (threadIdx.x == 0) L.b3 = P[OFF_B3];
+  __syncthreads();
+}
+
+// One tile: bx is this lane's B fragment of the normalised inputs (features
+// 8g..8g+7 of sample l%16, bf16).  Returns the prediction of sample l%16 in
+// every lane.
+__device__ __forceinline__ float infer_tile(const InferRegs &R, const InferLds &L, bf16x8 bx) {
+  const int l = threadIdx.x & 63, g = l >> 4;
+  bf16x8 bh[2];
+#pragma unroll
+  for (int nb = 0; nb < 4; ++nb) {
+    const f32x4 z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.a1[nb], bx, f32x4{0, 0, 0, 0}, 0, 0, 0);
+    const float4 bb = *reinterpret_cast<const float4 *>(&L.b1[nb * 16 + 4 * g]);
+    const float bv[4] = {bb.x, bb.y, bb.z, bb.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float u = z[i] + bv[i];
+      bh[nb >> 1][(nb & 1) * 4 + i] = (__bf16)(u * sigm(u));
+    }
+  }
+  float part = 0.f;
+#pragma unroll
+  for (int nb = 0; nb < 4; ++nb) {
+    f32x4 z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.a2[nb][0], bh[0], f32x4{0, 0, 0, 0}, 0, 0, 0);
+    z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.a2[nb][1], bh[1], z, 0, 0, 0);
+    const float4 bb = *reinterpret_cast<const float4 *>(&L.b2[nb * 16 + 4 * g]);
+    const float4 ww = *reinterpret_cast<const float4 *>(&L.w3[nb * 16 + 4 * g]);
+    const float bv[4] = {bb.x, bb.y, bb.z, bb.w}, wv[4] = {ww.x, ww.y, ww.z, ww.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float u = z[i] + bv[i];
+      part = fmaf(wv[i], u * sigm(u), part);
+    }
+  }
+  part += __shfl_xor(part, 16, 64);
+  part += __shfl_xor(part, 32, 64);
+  return part + L.b3;
+}
+
 __global__ __launch_bounds__(256) void mlp_forward_kernel(const float *__restrict__ P,
                                                           const float *__restrict__ norm,
                                                           const float *__restrict__ x, int64_t n,
                                                           float *__restrict__ y) {
-  __shared__ BlockLds L;
-  load_weights(L, P, norm);
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  WaveLds &W = L.wv[w];
-  const int64_t nwaves = (int64_t)gridDim.x * 4;
-  for (int64_t r0 = ((int64_t)blockIdx.x * 4 + w) * 16; r0 < n; r0 += nwaves * 16) {
-    stage_x(L, W, x, nullptr, r0, n);
-    f32x4 z1[4], z2[4];
-    forward16(L, W, z1, z2);
-    if (l < 16 && r0 + l < n) y[r0 + l] = W.y[l];
+  __shared__ InferLds L;
+  InferRegs R;
+  load_infer(P, norm, R, L);
+  const int l = threadIdx.x & 63, g = l >> 4, c = l & 15;
+  const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
+  for (int64_t r0 = wave * 16; r0 < n; r0 += nwaves * 16) {
+    const int64_t row = r0 + c;
+    bf16x8 bx;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int f = 8 * g + j;
+      const float v =
+          (row < n && f < FLK_MLP_NFEAT) ? (x[row * FLK_MLP_NFEAT + f] - L.mean[f]) * L.istd[f] : 0.f;
+      bx[j] = (__bf16)v;
+    }
+    const float yv = infer_tile(R, L, bx);
+    if (g == 0 && row < n) y[row] = yv;
+  }
+}
+
+// Learned criterion inside the traversal: predictions for the items of a
+// level chunk, straight from the node records.  Each wave takes 64 items per
+// iteration (one per lane): items whose decision cannot depend on the
+// prediction (two leaf buckets, or overlapping bounding spheres, which always
+// split) get +inf without the network; the others' normalised features are
+// compacted into the wave's LDS rows and evaluated 16 per tile.  The next
+// iteration's items and node records are loaded before the tiles run.
+__global__ __launch_bounds__(256) void bh_mlp_predict(const BhItem *__restrict__ items, int64_t n,
+                                                      const BhGeo *__restrict__ geo, float eps,
+                                                      const float *__restrict__ P,
+                                                      const float *__restrict__ norm,
+                                                      float *__restrict__ pred) {
+  __shared__ InferLds L;
+  __shared__ __attribute__((aligned(16))) __bf16 xs[4][64][16];
+  __shared__ int32_t src[4][64];
+  InferRegs R;
+  load_infer(P, norm, R, L);
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6, g = l >> 4, c = l & 15;
+  const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + w;
+  const int64_t stride = (int64_t)gridDim.x * (blockDim.x >> 6) * 64;
+  int64_t t = wave * 64 + l;
+  BhItem it = t < n ? items[t] : BhItem{0, 0, 0, 0};
+  BhGeo A = geo[it.a], B = geo[it.b];
+  for (int64_t base = wave * 64; base < n; base += stride, t += stride) {
+    bool need = false;
+    if (t < n) {
+      const float rx = (B.cx - A.cx) + (B.lx - A.lx), ry = (B.cy - A.cy) + (B.ly - A.ly),
+                  rz = (B.cz - A.cz) + (B.lz - A.lz);
+      const float dsq = fmaf(rx, rx, fmaf(ry, ry, rz * rz)) + eps;
+      const float rs = A.radius + B.radius;
+      need = !(A.c0 < 0 && B.c0 < 0) && dsq > rs * rs + eps;
+      if (need) {
+        float f[FLK_MLP_NFEAT];
+        mlp_features(A, B, dsq, f);
+        const unsigned long long m = __ballot(true);  // lanes with need == true
+        const int pos = __popcll(m & ((1ull << l) - 1ull));
+        bf16x8 v0, v1;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v0[j] = (__bf16)((f[j] - L.mean[j]) * L.istd[j]);
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          v1[j] = (__bf16)(j < FLK_MLP_NFEAT - 8 ? (f[8 + j] - L.mean[8 + j]) * L.istd[8 + j] : 0.f);
+        *reinterpret_cast<bf16x8 *>(&xs[w][pos][0]) = v0;
+        *reinterpret_cast<bf16x8 *>(&xs[w][pos][8]) = v1;
+        src[w][pos] = l;
+      } else {
+        pred[t] = __builtin_inff();
+      }
+    }
+    const int cnt = __popcll(__ballot(need));
     __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    // next iteration's records, in flight while the tiles run
+    const int64_t tn = t + stride;
+    if (tn < n) {
+      it = items[tn];
+      A = geo[it.a];
+      B = geo[it.b];
+    }
+    for (int r0 = 0; r0 < cnt; r0 += 16) {
+      const int row = r0 + c;
+      bf16x8 bx;
+      if (g < 2 && row < cnt) bx = *reinterpret_cast<const bf16x8 *>(&xs[w][row][8 * g]);
+      else
+#pragma unroll
+        for (int j = 0; j < 8; ++j) bx[j] = (__bf16)0.f;
+      const float yv = infer_tile(R, L, bx);
+      if (g == 0 && row < cnt) pred[base + src[w][row]] = yv;
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   }
 }
 
@@ -357,7 +525,7 @@ void mlp_forward(flk_ctx *ctx, const float *x, int64_t n, float *y) {
   dx.reserve(sizeof(float) * FLK_MLP_NFEAT * n);
   dy.reserve(sizeof(float) * n);
   FLK_HIP(hipMemcpyAsync(dx.ptr, x, sizeof(float) * FLK_MLP_NFEAT * n, hipMemcpyHostToDevice, s));
-  const int blocks = (int)std::min<int64_t>((n + 63) / 64, 4 * std::max(1, ctx->num_cus));
+  const int blocks = (int)std::min<int64_t>((n + 63) / 64, 8 * std::max(1, ctx->num_cus));
   hipLaunchKernelGGL(mlp_forward_kernel, dim3(blocks), dim3(256), 0, s, ctx->mlp.d_params.as<float>(),
                      ctx->mlp.d_norm.as<float>(), dx.as<float>(), n, dy.as<float>());
   FLK_HIP(hipGetLastError());
@@ -366,9 +534,18 @@ void mlp_forward(flk_ctx *ctx, const float *x, int64_t n, float *y) {
 }
 
 void mlp_forward_device(flk_ctx *ctx, const float *dx, int64_t n, float *dy) {
-  const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((n + 63) / 64, 4 * std::max(1, ctx->num_cus)));
+  const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((n + 63) / 64, 8 * std::max(1, ctx->num_cus)));
   hipLaunchKernelGGL(mlp_forward_kernel, dim3(blocks), dim3(256), 0, ctx->stream,
                      ctx->mlp.d_params.as<float>(), ctx->mlp.d_norm.as<float>(), dx, n, dy);
+  FLK_HIP(hipGetLastError());
+}
+
+void mlp_predict_items(flk_ctx *ctx, const BhItem *items, int64_t n, const BhGeo *geo, float eps,
+                       float *pred) {
+  if (n <= 0) return;
+  const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8 * std::max(1, ctx->num_cus)));
+  hipLaunchKernelGGL(bh_mlp_predict, dim3(blocks), dim3(256), 0, ctx->stream, items, n, geo, eps,
+                     ctx->mlp.d_params.as<float>(), ctx->mlp.d_norm.as<float>(), pred);
   FLK_HIP(hipGetLastError());
 }
 
@@ -387,7 +564,7 @@ float mlp_gradient(flk_ctx *ctx, const float *x, const float *y, int64_t n, floa
   hipLaunchKernelGGL(zero_kernel, dim3((FLK_MLP_NPARAM + 255) / 256), dim3(256), 0, s, dg.as<float>(),
                      FLK_MLP_NPARAM);
   hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, s, dl.as<float>(), 1);
-  const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((n + 63) / 64, 4 * std::max(1, ctx->num_cus)));
+  const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((n + 63) / 64, 8 * std::max(1, ctx->num_cus)));
   hipLaunchKernelGGL(mlp_grad_kernel, dim3(blocks), dim3(256), 0, s, ctx->mlp.d_params.as<float>(),
                      ctx->mlp.d_norm.as<float>(), dx.as<float>(), dt.as<float>(), nullptr, n,
                      1.0f / (float)std::max<int64_t>(n, 1), dg.as<float>(), dl.as<float>());

@@ -32,16 +32,20 @@ __device__ __forceinline__ bool box_overlap(const Box6 &a, const Box6 &b) {
          b.mn[1] <= a.mx[1] && b.mn[2] <= a.mx[2];
 }
 
-// One block per curve: FP64 min/max over its vertices (the polyline case of
-// reference curve.cpp:331-343; an empty curve gets the empty box).
-__global__ __launch_bounds__(256) void curve_boxes_kernel(const double *__restrict__ xyz,
-                                                          const int64_t *__restrict__ off,
-                                                          int32_t ncurves,
-                                                          double *__restrict__ boxes) {
-  const int c = blockIdx.x;
-  if (c >= ncurves) return;
+// FP64 min/max of each curve's vertices (the polyline case of reference
+// curve.cpp:331-343; an empty curve gets the empty box) in two passes: one
+// block per chunk of at most kBoxChunk vertices of one curve, then one thread
+// per curve folding its chunks, so a curve of millions of vertices is spread
+// over the whole chip.
+constexpr int64_t kBoxChunk = 16384;
+
+__global__ __launch_bounds__(256) void curve_box_chunks(const double *__restrict__ xyz,
+                                                        const int64_t *__restrict__ chunk_v,
+                                                        double *__restrict__ part) {
+  const int64_t c = blockIdx.x;
+  const int64_t v0 = chunk_v[2 * c], v1 = chunk_v[2 * c + 1];
   double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
-  for (int64_t v = off[c] + threadIdx.x; v < off[c + 1]; v += blockDim.x) {
+  for (int64_t v = v0 + threadIdx.x; v < v1; v += blockDim.x) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       const double x = xyz[3 * v + d];
@@ -65,14 +69,31 @@ __global__ __launch_bounds__(256) void curve_boxes_kernel(const double *__restri
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int d = 0; d < 3; ++d) {
-      double a = smn[d][0], b = smx[d][0];
+      double lo = smn[d][0], hi = smx[d][0];
       for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
-        a = fmin(a, smn[d][k]);
-        b = fmax(b, smx[d][k]);
+        lo = fmin(lo, smn[d][k]);
+        hi = fmax(hi, smx[d][k]);
       }
-      boxes[6 * c + d] = a;
-      boxes[6 * c + 3 + d] = b;
+      part[6 * c + d] = lo;
+      part[6 * c + 3 + d] = hi;
     }
+  }
+}
+
+__global__ __launch_bounds__(256) void curve_box_merge(const double *__restrict__ part,
+                                                       const int64_t *__restrict__ coff, int32_t ncurves,
+                                                       double *__restrict__ boxes) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncurves) return;
+  double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+  for (int64_t k = coff[c]; k < coff[c + 1]; ++k)
+    for (int d = 0; d < 3; ++d) {
+      mn[d] = fmin(mn[d], part[6 * k + d]);
+      mx[d] = fmax(mx[d], part[6 * k + 3 + d]);
+    }
+  for (int d = 0; d < 3; ++d) {
+    boxes[6 * c + d] = mn[d];
+    boxes[6 * c + 3 + d] = mx[d];
   }
 }
 
@@ -212,16 +233,35 @@ void curve_boxes_run(flk_ctx *ctx, std::vector<double> &boxes) {
   Curves &cv = ctx->curves;
   boxes.assign(6 * (size_t)cv.ncurves, 0.0);
   if (cv.ncurves == 0) return;
-  ctx->d_boxes.reserve(sizeof(double) * 6 * cv.ncurves);
-  ctx->d_misc.reserve(sizeof(int64_t) * (cv.ncurves + 1));
-  FLK_HIP(hipMemcpyAsync(ctx->d_misc.ptr, cv.in_off.data(), sizeof(int64_t) * (cv.ncurves + 1),
-                         hipMemcpyHostToDevice, ctx->stream));
-  hipLaunchKernelGGL(curve_boxes_kernel, dim3(cv.ncurves), dim3(256), 0, ctx->stream,
-                     cv.raw.as<double>(), ctx->d_misc.as<int64_t>(), cv.ncurves,
-                     ctx->d_boxes.as<double>());
+  // chunk table: (first, end) vertex of every chunk, and each curve's chunks
+  std::vector<int64_t> meta(cv.ncurves + 1);
+  std::vector<int64_t> chunk_v;
+  for (int32_t c = 0; c < cv.ncurves; ++c) {
+    meta[c] = (int64_t)chunk_v.size() / 2;
+    for (int64_t v = cv.in_off[c]; v < cv.in_off[c + 1]; v += kBoxChunk) {
+      chunk_v.push_back(v);
+      chunk_v.push_back(std::min(v + kBoxChunk, cv.in_off[c + 1]));
+    }
+  }
+  const int64_t nchunks = (int64_t)chunk_v.size() / 2;
+  meta[cv.ncurves] = nchunks;
+  meta.insert(meta.end(), chunk_v.begin(), chunk_v.end());
+  ctx->d_boxes.reserve(sizeof(double) * 6 * (cv.ncurves + std::max<int64_t>(nchunks, 1)));
+  ctx->d_misc.reserve(sizeof(int64_t) * meta.size());
+  FLK_HIP(hipMemcpyAsync(ctx->d_misc.ptr, meta.data(), sizeof(int64_t) * meta.size(), hipMemcpyHostToDevice,
+                         ctx->stream));
+  double *d_boxes = ctx->d_boxes.as<double>(), *d_part = d_boxes + 6 * (size_t)cv.ncurves;
+  const int64_t *d_coff = ctx->d_misc.as<int64_t>(), *d_chunk = d_coff + cv.ncurves + 1;
+  if (nchunks > 0) {
+    hipLaunchKernelGGL(curve_box_chunks, dim3((unsigned)nchunks), dim3(256), 0, ctx->stream,
+                       cv.raw.as<double>(), d_chunk, d_part);
+    FLK_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(curve_box_merge, dim3((unsigned)((cv.ncurves + 255) / 256)), dim3(256), 0, ctx->stream,
+                     d_part, d_coff, cv.ncurves, d_boxes);
   FLK_HIP(hipGetLastError());
-  FLK_HIP(hipMemcpyAsync(boxes.data(), ctx->d_boxes.ptr, sizeof(double) * 6 * cv.ncurves,
-                         hipMemcpyDeviceToHost, ctx->stream));
+  FLK_HIP(hipMemcpyAsync(boxes.data(), d_boxes, sizeof(double) * 6 * cv.ncurves, hipMemcpyDeviceToHost,
+                         ctx->stream));
   FLK_HIP(hipStreamSynchronize(ctx->stream));
 }
 

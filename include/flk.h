@@ -57,7 +57,10 @@ typedef struct flk_params {
   int32_t fp64;         /* 1: direct sum in FP64 instead of FP32 */
   double mlp_log_tol;   /* MLP criterion: accept far field when the predicted
                            log10 |error| is below this */
-  int32_t reserved[8];
+  int32_t bh_batch_ksegs; /* Barnes–Hut pairs are processed in batches whose
+                             curves total at most this many thousand segments
+                             (0: sized from free device memory) */
+  int32_t reserved[7];
 } flk_params;
 
 typedef struct flk_stats {

@@ -89,3 +89,31 @@ def test_bh_direct_threshold_and_retry(engine):
     once = engine.linking_numbers([[0, 1]], method="barneshut", fallback_tol=0.0, beta_limit=2.0)
     assert once.stats["retries"] == 0
     assert abs(res.lk[0] - ref[0]) <= abs(once.lk[0] - ref[0])
+
+
+def test_bh_lattice_hybrid_batches(engine):
+    """Helical lattice at 1/50 size: the direct-sum/Barnes–Hut hybrid over
+    every candidate pair, with the Barnes–Hut pairs in one batch and in
+    batches of at most 20k segments (curves rebuilt per batch), against the
+    FP32 direct sum of every pair (whose integers the oracle pins elsewhere)."""
+    curves = W.workload(4, scale=0.02)
+    engine.set_curves(curves)
+    pairs = engine.potential_links()
+    sizes = np.array([len(c) for c in curves])
+    thr = 512 * 512
+    nbh = int(np.sum(sizes[pairs[:, 0]] * sizes[pairs[:, 1]] >= thr))
+    assert 10 < nbh < len(pairs)
+    ds = engine.linking_numbers(pairs, method="directsum", fallback_tol=0.0)
+    one = engine.linking_numbers(pairs, method="barneshut", bh_direct_threshold=thr, fallback_tol=0.0)
+    many = engine.linking_numbers(pairs, method="barneshut", bh_direct_threshold=thr, fallback_tol=0.0,
+                                  bh_batch_ksegs=20)
+    print(f"{len(curves)} curves, {len(pairs)} pairs, {nbh} BH; max |bh - ds| "
+          f"{np.max(np.abs(one.lk - ds.lk)):.3g}, batched {np.max(np.abs(many.lk - ds.lk)):.3g}")
+    direct = (one.flags & flk.FLK_PAIR_DIRECT) != 0
+    assert int(np.sum(~direct)) == nbh
+    assert np.all(np.round(one.lk) == np.round(ds.lk)) and np.all(np.round(many.lk) == np.round(ds.lk))
+    assert np.max(np.abs(one.lk - ds.lk)) < 0.05
+    # batching changes only which trees are built together, not any sum
+    np.testing.assert_allclose(many.lk, one.lk, rtol=0, atol=1e-9)
+    assert one.stats["bh_visits"] == many.stats["bh_visits"]
+    assert flk.certificate_from_lk(pairs, many.lk) == flk.certificate_from_lk(pairs, ds.lk)
