@@ -10,9 +10,13 @@ timing (resident in HBM).
 
 Prints one JSON line.  `roofline` is for the direct-sum tile kernel
 (58 algorithmic FLOP per segment pair, against the 157.3 TFLOP/s FP32 peak),
-timed with HIP events inside the engine on its own stream.  `cpu_baseline`
-times the oracle's restatement of the reference's OpenMP direct-sum pair loop
-on a deterministic sample of the same candidate pairs on this host.
+timed with HIP events inside the engine on its own stream; with --method
+barneshut it is for the traversal level kernel (336 canonical bytes per
+node-pair visit against 8 TB/s).  `traffic` is the HBM bytes per launch that
+rocprofv3 measured for the same workload (profiles/traffic.json).
+`cpu_baseline` times the oracle's restatement of the reference's OpenMP
+direct-sum pair loop on a deterministic sample of the same candidate pairs on
+this host (with --method barneshut: the reference Barnes-Hut, whole workload).
 """
 import argparse
 import json
@@ -56,9 +60,6 @@ def main():
                     help="time the full workload-5 direct sum instead of a 1/16 row sample")
     ap.add_argument("--criterion", default="beta", choices=["beta", "mlp"],
                     help="Barnes-Hut opening criterion for --method barneshut")
-    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
-                    help="also run a device-to-device copy of MIB mebibytes (a known-size "
-                         "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
@@ -82,14 +83,6 @@ def main():
     if args.criterion == "mlp":
         from fastlinkingnumbers_amd import criterion
         eng.mlp_set_weights(*criterion.load_weights())
-    if args.calibrate_copy:
-        import torch  # device memory plumbing for the calibration copy only
-        src = torch.empty(args.calibrate_copy << 18, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        log(f"calibration copy: {args.calibrate_copy} MiB read + {args.calibrate_copy} MiB written")
-        del src, dst
 
     def step():
         pairs = eng.potential_links()
@@ -115,29 +108,40 @@ def main():
 
     ms_kernel = float(np.mean(kern))
     achieved = FLOP_PER_SEG_PAIR * st["ds_seg_pairs"] / (ms_kernel * 1e-3) * 1e-12 if ms_kernel > 0 else None
+    measured = profiled_traffic()
     if args.method == "barneshut":
-        gbps = st["bh_visits"] * BH_BYTES_PER_VISIT / (st["ms_bh_traverse"] * 1e-3) * 1e-9
+        # canonical bytes per visit over the level kernels' device time: HIP
+        # events on every level launch of one more certificate after the timed
+        # steps (per-launch events slow a traversal by ~15%, so the timed
+        # steps run without them)
+        tparams = flk.default_params(method=args.method, criterion=args.criterion, timing=1)
+        ms_level = eng.linking_numbers(pairs, tparams).stats["ms_bh_level"]
+        gbps = st["bh_visits"] * BH_BYTES_PER_VISIT / (ms_level * 1e-3) * 1e-9
+        t = measured.get("bh_level/random_walks_2x8M_beta") if (args.workload, args.scale, args.criterion) == (5, 1.0, "beta") else None
         roofline = {
-            "bound": "hbm", "kernel": "bh_level (+bh_leaf_pairs, far field inline)",
+            "bound": "hbm", "kernel": "bh_level (far field inline), all launches of one certificate",
             "achieved": round(gbps, 1), "peak": 8000.0, "unit": "GB/s",
-            "frac": round(gbps / 8000.0, 4), "traffic": None,
+            "frac": round(gbps / 8000.0, 4),
+            "traffic": t["bytes_per_certificate"] if t else None,
+            "traffic_GBps": round(t["bytes_per_certificate"] / (ms_level * 1e-3) * 1e-9, 1) if t else None,
             "bytes_per_unit": BH_BYTES_PER_VISIT, "units": int(st["bh_visits"]),
-            "ms_traverse": round(st["ms_bh_traverse"], 3),
+            "ms_kernel": round(ms_level, 3), "launches": "one per level chunk",
+            "kernel_timing": "HIP events on each level launch of a certificate run after the timed steps",
         }
     else:
-        roofline = None
-    roofline = roofline or {
-        "bound": "valu",
-        "kernel": "ds_tile_f32",
-        "achieved": round(achieved, 3) if achieved else None,
-        "peak": FP32_PEAK_TFLOPS,
-        "unit": "TFLOP/s",
-        "frac": round(achieved / FP32_PEAK_TFLOPS, 4) if achieved else None,
-        "traffic": None,
-        "flop_per_unit": FLOP_PER_SEG_PAIR,
-        "units_per_launch": int(st["ds_seg_pairs"]),
-        "ms_per_launch": round(ms_kernel, 3),
-    }
+        t = measured.get("ds_tile_f32/chainmail1600") if (args.workload, args.scale) == (2, 1.0) else None
+        roofline = {
+            "bound": "valu",
+            "kernel": "ds_tile_f32",
+            "achieved": round(achieved, 3) if achieved else None,
+            "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / FP32_PEAK_TFLOPS, 4) if achieved else None,
+            "traffic": t["bytes_per_launch"] if t else None,
+            "flop_per_unit": FLOP_PER_SEG_PAIR,
+            "units_per_launch": int(st["ds_seg_pairs"]),
+            "ms_per_launch": round(ms_kernel, 3),
+        }
 
     cpu = None
     if args.cpu_seconds > 0 and len(pairs) and args.method == "barneshut":
@@ -224,6 +228,16 @@ def main():
 BH_BYTES_PER_VISIT = 336  # two 160-byte node records + 16 bytes of queue traffic
 
 
+def profiled_traffic():
+    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+    (calibrated) for the profiled workloads, from profiles/traffic.json."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return {}
+    with open(path) as fh:
+        return json.load(fh)
+
+
 def learned_vs_directsum(eng, flk, workloads, args):
     """Workload 5: two interlinked 8M-vertex random-walk loops.  Times the GPU
     FP32 direct sum (a 1/16 row sample scaled by 16 unless --full-ds: the
@@ -256,13 +270,14 @@ def learned_vs_directsum(eng, flk, workloads, args):
         eng.linking_numbers([[0, 1]], method="barneshut", **kw)  # warm-up
         r = eng.linking_numbers([[0, 1]], method="barneshut", **kw)
         st = r.stats
+        dev = eng.linking_numbers([[0, 1]], method="barneshut", timing=1, **kw).stats  # kernel times
         res[name] = {"ms": round(st["ms_total"], 1), "lk": float(r.lk[0]),
                      "off_integer": float(abs(r.lk[0] - round(r.lk[0]))),
                      "visits": int(st["bh_visits"]), "far": int(st["bh_far"]),
                      "leaf_pairs": int(st["bh_leaf_pairs"]), "retries": int(st["retries"]),
                      "fallbacks": int(st["fallbacks"]),
-                     "traverse_GBps": round(st["bh_visits"] * BH_BYTES_PER_VISIT /
-                                            (st["ms_bh_traverse"] * 1e-3) * 1e-9, 1)}
+                     "ms_build": round(st["ms_bh_build"], 2), "ms_traverse": round(st["ms_bh_traverse"], 2),
+                     "ms_level_kernels": round(dev["ms_bh_level"], 2), "ms_mlp_kernels": round(dev["ms_mlp"], 2)}
         log(f"workload 5 {name}: {res[name]}")
     res["learned_speedup_vs_ds"] = round(res["ds_ms"] / res["learned"]["ms"], 1)
     res["learned_speedup_vs_beta"] = round(res["beta"]["ms"] / res["learned"]["ms"], 2)

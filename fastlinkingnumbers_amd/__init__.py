@@ -48,7 +48,8 @@ class FlkParams(C.Structure):
         ("fp64", C.c_int32),
         ("mlp_log_tol", C.c_double),
         ("bh_batch_ksegs", C.c_int32),
-        ("reserved", C.c_int32 * 7),
+        ("timing", C.c_int32),
+        ("reserved", C.c_int32 * 6),
     ]
 
 
@@ -69,7 +70,9 @@ class FlkStats(C.Structure):
         ("retries", C.c_int64),
         ("ds_launches", C.c_int32),
         ("ds_waves", C.c_int32),
-        ("reserved", C.c_int64 * 8),
+        ("ms_bh_level", C.c_double),
+        ("ms_mlp", C.c_double),
+        ("reserved", C.c_int64 * 6),
     ]
 
     def as_dict(self) -> dict:

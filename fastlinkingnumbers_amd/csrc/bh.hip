@@ -889,7 +889,7 @@ void build_forest(flk_ctx *ctx, const std::vector<int32_t> &curves_in, int leaf,
 // rotate through a pre-zeroed array instead of a clear per launch.
 void traverse(flk_ctx *ctx, Forest &F, const std::vector<int32_t> &ra, const std::vector<int32_t> &rb,
               const std::vector<float> &beta2, float eps, double *d_sum, double *d_err,
-              unsigned long long stats[3], bool use_mlp = false, float log_tol = 0.f) {
+              unsigned long long stats[3], bool use_mlp, float log_tol, bool timing) {
   hipStream_t s = ctx->stream;
   BhScratch &B = ctx->bh;
   const int64_t np = (int64_t)ra.size();
@@ -945,21 +945,30 @@ void traverse(flk_ctx *ctx, Forest &F, const std::vector<int32_t> &ra, const std
       FLK_HIP(hipStreamSynchronize(s));  // the previous leaf launch may still read it
       B.leaves.reserve(sizeof(BhItem) * take);
     }
-    if (use_mlp) {
-      B.pred.reserve(sizeof(float) * take);
-      mlp_predict_items(ctx, src, take, F.geo.as<BhGeo>(), eps, B.pred.as<float>());
-    }
+    if (use_mlp) B.pred.reserve(sizeof(float) * take);
     const int slot = (int)(iter % kSlots);
     if (slot == 0 && iter > 0)
       FLK_HIP(hipMemsetAsync(cnt + 4, 0, sizeof(unsigned long long) * 2 * kSlots, s));
+    if (use_mlp)
+      mlp_predict_items(ctx, src, take, F.geo.as<BhGeo>(), eps, B.pred.as<float>(), timing ? ctx->ev0 : nullptr,
+                        timing ? ctx->ev3 : nullptr);
     unsigned long long *qc = cnt + 4 + 2 * slot;  // [0] next items, [1] leaf items
-    hipLaunchKernelGGL(bh_level, dim3((unsigned)((take + kTravItems - 1) / kTravItems)), dim3(kTravBlock),
-                       0, s, src, take, F.geo.as<BhGeo>(), F.mom.as<BhMom>(), B.beta2.as<float>(), eps,
-                       use_mlp ? B.pred.as<float>() : nullptr, log_tol, spare->as<BhItem>(), qc,
-                       2 * take, B.leaves.as<BhItem>(), qc + 1, take, d_sum, d_err);
+    // with timing, start/stop events ride on the dispatch itself
+    launch_timed(timing ? ctx->ev1 : nullptr, timing ? ctx->ev2 : nullptr, bh_level,
+                 dim3((unsigned)((take + kTravItems - 1) / kTravItems)), dim3(kTravBlock), s, src, take,
+                 F.geo.as<BhGeo>(), F.mom.as<BhMom>(), B.beta2.as<float>(), eps,
+                 use_mlp ? B.pred.as<float>() : nullptr, log_tol, spare->as<BhItem>(), qc, 2 * take,
+                 B.leaves.as<BhItem>(), qc + 1, take, d_sum, d_err);
     FLK_HIP(hipGetLastError());
     FLK_HIP(hipMemcpyAsync(B.host_cnt, qc, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     FLK_HIP(hipStreamSynchronize(s));
+    if (timing) {
+      float ms_pred = 0.f, ms_level = 0.f;
+      if (use_mlp) FLK_HIP(hipEventElapsedTime(&ms_pred, ctx->ev0, ctx->ev3));
+      FLK_HIP(hipEventElapsedTime(&ms_level, ctx->ev1, ctx->ev2));
+      ctx->stats.ms_mlp += ms_pred;
+      ctx->stats.ms_bh_level += ms_level;
+    }
     ++iter;
     const unsigned long long nn[2] = {B.host_cnt[0], B.host_cnt[1]};
     if ((int64_t)nn[0] > 2 * take || (int64_t)nn[1] > take)
@@ -1095,7 +1104,7 @@ void bh_batch(flk_ctx *ctx, const int32_t *pairs, const std::vector<int64_t> &bh
   FLK_HIP(hipMemsetAsync(d_err, 0, sizeof(double) * 2 * nb, s));
   unsigned long long st[3] = {0, 0, 0};
   t0 = std::chrono::steady_clock::now();
-  traverse(ctx, F, ra, rb, beta2, eps, d_sum, d_err, st, use_mlp, (float)prm.mlp_log_tol);
+  traverse(ctx, F, ra, rb, beta2, eps, d_sum, d_err, st, use_mlp, (float)prm.mlp_log_tol, prm.timing != 0);
   std::vector<double> v(nb), e(nb);
   FLK_HIP(hipMemcpyAsync(v.data(), d_sum, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
   FLK_HIP(hipMemcpyAsync(e.data(), d_err, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
@@ -1132,7 +1141,8 @@ void bh_batch(flk_ctx *ctx, const int32_t *pairs, const std::vector<int64_t> &bh
   if (!redo.empty()) {
     const int64_t nr = (int64_t)redo.size();
     double *d_sum2 = d_sum + nb, *d_err2 = d_err + nb;
-    traverse(ctx, F, ra2, rb2, b2, eps, d_sum2, d_err2, st, use_mlp, (float)prm.mlp_log_tol - 1.0f);
+    traverse(ctx, F, ra2, rb2, b2, eps, d_sum2, d_err2, st, use_mlp, (float)prm.mlp_log_tol - 1.0f,
+             prm.timing != 0);
     std::vector<double> v2(nr), e2(nr);
     FLK_HIP(hipMemcpyAsync(v2.data(), d_sum2, sizeof(double) * nr, hipMemcpyDeviceToHost, s));
     FLK_HIP(hipMemcpyAsync(e2.data(), d_err2, sizeof(double) * nr, hipMemcpyDeviceToHost, s));

@@ -24,8 +24,6 @@
 // deterministic.
 #include <hip/hip_runtime.h>
 
-#include <cfloat>
-
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -188,7 +186,7 @@ __global__ __launch_bounds__(256) void ds_tile_f32(
         diag[k] = fmaf(gx[k], nx[k + 1], fmaf(gy[k], ny[k + 1], gz[k] * nz[k + 1]));
       }
       float xp[R], yp[R], s12[R], dt[R];
-      float worst = -FLT_MAX, smin = FLT_MAX;  // max_k |y'| - x'/8, min_k (D1 + D2)
+      bool small = true;
 #pragma unroll
       for (int k = 0; k < R; ++k) {
         const float la = gl[k], lb = gl[k + 1], lc = nl[k + 1], ld = nl[k];
@@ -203,11 +201,12 @@ __global__ __launch_bounds__(256) void ds_tile_f32(
         xp[k] = fmaf(div1, div2, -dt[k] * dt[k]);
         s12[k] = div1 + div2;
         yp[k] = dt[k] * s12[k];
-        worst = fmaxf(worst, fmaf(-0.125f, xp[k], fabsf(yp[k])));
-        smin = fminf(smin, s12[k]);
+        // small angle: x' > 0, no branch correction, |y'/x'| <= 1/8 (per-pair
+        // tests interleave with the arithmetic; a max/min fold at the end of
+        // the step serialises and measured 9% slower)
+        small = small && (s12[k] > 0.f) && (fmaf(0.125f, xp[k], -fabsf(yp[k])) > 0.f);
       }
-      // small angle for all R pairs: x' > 0, no branch correction, |y'/x'| < 1/8
-      if (__all(worst < 0.f && smin > 0.f)) {
+      if (__all(small)) {
         // atan(t) = t - t^3/3 + t^5/5 - t^7/7 for |t| <= 1/8 (error < 1e-9)
 #pragma unroll
         for (int k = 0; k < R; ++k) {

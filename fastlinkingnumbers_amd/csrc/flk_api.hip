@@ -72,6 +72,8 @@ int32_t flk_create(int32_t device, flk_ctx **out) {
     FLK_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     FLK_HIP(hipEventCreate(&ctx->ev0));
     FLK_HIP(hipEventCreate(&ctx->ev1));
+    FLK_HIP(hipEventCreate(&ctx->ev2));
+    FLK_HIP(hipEventCreate(&ctx->ev3));
   } catch (const flk::Error &e) {
     g_create_err = e.msg;
     delete ctx;
@@ -95,6 +97,8 @@ void flk_destroy(flk_ctx *ctx) {
   ctx->bh.release();
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  if (ctx->ev2) (void)hipEventDestroy(ctx->ev2);
+  if (ctx->ev3) (void)hipEventDestroy(ctx->ev3);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }

@@ -1,6 +1,7 @@
 // Internal declarations of the MI355X linking-number engine (libflk.so).
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -27,6 +28,15 @@ struct Error {
 };
 
 // Device allocation that grows on demand and is freed with its owner.
+// Kernel launch that optionally carries start/stop events on the dispatch
+// itself (hipExtLaunchKernelGGL); without events it is a plain launch.
+template <typename K, typename... Args>
+inline void launch_timed(hipEvent_t start, hipEvent_t stop, K kernel, dim3 grid, dim3 block, hipStream_t s,
+                         Args... args) {
+  if (start && stop) hipExtLaunchKernelGGL(kernel, grid, block, 0, s, start, stop, 0, args...);
+  else hipLaunchKernelGGL(kernel, grid, block, 0, s, args...);
+}
+
 struct DevBuf {
   void *ptr = nullptr;
   size_t bytes = 0;
@@ -155,7 +165,7 @@ struct MlpState {
 struct flk_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   int num_cus = 0;
   std::string name;
   std::string err;
@@ -197,7 +207,7 @@ void mlp_forward_device(flk_ctx *ctx, const float *dx, int64_t n, float *dy);
 struct BhItem;
 struct BhGeo;
 void mlp_predict_items(flk_ctx *ctx, const BhItem *items, int64_t n, const BhGeo *geo, float eps,
-                       float *pred);
+                       float *pred, hipEvent_t start, hipEvent_t stop);
 float mlp_gradient(flk_ctx *ctx, const float *x, const float *y, int64_t n, float *grad);
 float mlp_train(flk_ctx *ctx, const float *x, const float *y, int64_t n, int epochs, int batch,
                 float lr, uint64_t seed);

@@ -541,11 +541,11 @@ void mlp_forward_device(flk_ctx *ctx, const float *dx, int64_t n, float *dy) {
 }
 
 void mlp_predict_items(flk_ctx *ctx, const BhItem *items, int64_t n, const BhGeo *geo, float eps,
-                       float *pred) {
+                       float *pred, hipEvent_t start, hipEvent_t stop) {
   if (n <= 0) return;
   const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8 * std::max(1, ctx->num_cus)));
-  hipLaunchKernelGGL(bh_mlp_predict, dim3(blocks), dim3(256), 0, ctx->stream, items, n, geo, eps,
-                     ctx->mlp.d_params.as<float>(), ctx->mlp.d_norm.as<float>(), pred);
+  launch_timed(start, stop, bh_mlp_predict, dim3(blocks), dim3(256), ctx->stream, items, n, geo, eps,
+               ctx->mlp.d_params.as<float>(), ctx->mlp.d_norm.as<float>(), pred);
   FLK_HIP(hipGetLastError());
 }
 

@@ -60,15 +60,18 @@ typedef struct flk_params {
   int32_t bh_batch_ksegs; /* Barnes–Hut pairs are processed in batches whose
                              curves total at most this many thousand segments
                              (0: sized from free device memory) */
-  int32_t reserved[7];
+  int32_t timing;       /* 1: time every Barnes–Hut level and prediction launch
+                           with HIP events (ms_bh_level, ms_mlp); the per-launch
+                           events cost ~15% of a traversal, so 0 by default */
+  int32_t reserved[6];
 } flk_params;
 
 typedef struct flk_stats {
   double ms_total;          /* wall time of the last flk_linking_numbers call */
   double ms_kernel;         /* device time of its main kernel(s), HIP events */
   double ms_ds_kernel;      /* device time of the direct-sum tile kernel */
-  double ms_bh_traverse;    /* device time of the Barnes–Hut traversal */
-  double ms_bh_build;       /* device time of tree + moment construction */
+  double ms_bh_traverse;    /* wall time of the Barnes–Hut traversal (host clock) */
+  double ms_bh_build;       /* wall time of tree + moment construction (host clock) */
   double ms_fallback;       /* device time of FP64 fallback recomputation */
   int64_t seg_pairs;        /* sum of n1*n2 over the requested pairs */
   int64_t ds_seg_pairs;     /* segment pairs evaluated by direct summation */
@@ -79,7 +82,9 @@ typedef struct flk_stats {
   int64_t retries;          /* pairs re-run with a larger beta */
   int32_t ds_launches;      /* direct-sum tile kernel launches */
   int32_t ds_waves;         /* waves per direct-sum launch */
-  int64_t reserved[8];
+  double ms_bh_level;       /* device time of the traversal level kernels (params.timing) */
+  double ms_mlp;            /* device time of the learned-criterion predictions (params.timing) */
+  int64_t reserved[6];
 } flk_stats;
 
 /* Fills *p with the reference defaults (method Barnes–Hut, beta 2/10, ...). */

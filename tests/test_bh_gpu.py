@@ -117,3 +117,20 @@ def test_bh_lattice_hybrid_batches(engine):
     np.testing.assert_allclose(many.lk, one.lk, rtol=0, atol=1e-9)
     assert one.stats["bh_visits"] == many.stats["bh_visits"]
     assert flk.certificate_from_lk(pairs, many.lk) == flk.certificate_from_lk(pairs, ds.lk)
+
+
+@pytest.mark.parametrize("criterion", ["beta", "mlp"])
+def test_bh_kernel_timing_mode(engine, criterion):
+    """params.timing = 1 times every level (and prediction) launch with HIP
+    events without changing what is computed."""
+    from fastlinkingnumbers_amd import criterion as crit
+    engine.mlp_set_weights(*crit.load_weights())
+    curves = W.random_walk_pair(seed=3, n=50000, rg=40.0, workload=3)
+    engine.set_curves(curves)
+    plain = engine.linking_numbers([[0, 1]], method="barneshut", criterion=criterion, fallback_tol=0.0)
+    timed = engine.linking_numbers([[0, 1]], method="barneshut", criterion=criterion, fallback_tol=0.0, timing=1)
+    assert plain.stats["ms_bh_level"] == 0.0 and plain.stats["ms_mlp"] == 0.0
+    assert 0.0 < timed.stats["ms_bh_level"] < timed.stats["ms_total"]
+    assert (timed.stats["ms_mlp"] > 0.0) == (criterion == "mlp")
+    assert timed.stats["bh_visits"] == plain.stats["bh_visits"]
+    np.testing.assert_allclose(timed.lk, plain.lk, rtol=0, atol=1e-9)
