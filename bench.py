@@ -19,6 +19,7 @@ direct-sum pair loop on a deterministic sample of the same candidate pairs on
 this host (with --method barneshut: the reference Barnes-Hut, whole workload).
 """
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -58,6 +59,9 @@ def main():
                     help="also run learned BH vs GPU direct sum on workload 5 (2x8M)")
     ap.add_argument("--full-ds", action="store_true",
                     help="time the full workload-5 direct sum instead of a 1/16 row sample")
+    ap.add_argument("--direct-threshold", type=int, default=None,
+                    help="Barnes-Hut pairs with n1*n2 below this use the direct sum "
+                         "(default 4096^2 for workload 4, else 0)")
     ap.add_argument("--criterion", default="beta", choices=["beta", "mlp"],
                     help="Barnes-Hut opening criterion for --method barneshut")
     args = ap.parse_args()
@@ -79,15 +83,24 @@ def main():
     eng = flk.Engine(0)
     log("device:", eng.device_name())
     eng.set_curves_packed(xyz, offs)
-    params = flk.default_params(method=args.method, criterion=args.criterion)
+    if args.direct_threshold is None:
+        args.direct_threshold = 4096 * 4096 if args.workload == 4 else 0
+    params = flk.default_params(method=args.method, criterion=args.criterion,
+                                bh_direct_threshold=args.direct_threshold)
     if args.criterion == "mlp":
         from fastlinkingnumbers_amd import criterion
         eng.mlp_set_weights(*criterion.load_weights())
 
+    phase = {}
+
     def step():
+        t = time.perf_counter()
         pairs = eng.potential_links()
+        phase["pls"] = time.perf_counter() - t
         res = eng.linking_numbers(pairs, params)
+        t = time.perf_counter()
         cert = flk.certificate_from_lk(pairs, res.lk)
+        phase["round"] = time.perf_counter() - t
         return pairs, res, cert
 
     for _ in range(args.warmup):
@@ -98,8 +111,11 @@ def main():
         pairs, res, cert = step()
         times.append(time.perf_counter() - t)
         kern.append(res.stats["ms_ds_kernel"])
-        log(f"step {len(times)}: {times[-1] * 1e3:.1f} ms (direct-sum kernel {kern[-1]:.1f} ms, "
-            f"bh traverse {res.stats['ms_bh_traverse']:.1f} ms, visits {res.stats['bh_visits']})")
+        log(f"step {len(times)}: {times[-1] * 1e3:.1f} ms (engine {res.stats['ms_total']:.1f} ms: direct-sum "
+            f"kernel {kern[-1]:.1f} ms, bh build {res.stats['ms_bh_build']:.1f} ms, traverse "
+            f"{res.stats['ms_bh_traverse']:.1f} ms, visits {res.stats['bh_visits']}, fallback "
+            f"{res.stats['ms_fallback']:.1f} ms; pls {phase['pls'] * 1e3:.1f} ms, rounding "
+            f"{phase['round'] * 1e3:.1f} ms)")
     st = res.stats
     seg_pairs = int(np.sum(sizes[pairs[:, 0]] * sizes[pairs[:, 1]])) if len(pairs) else 0
     ms_step = 1e3 * float(np.mean(times))
@@ -114,7 +130,8 @@ def main():
         # events on every level launch of one more certificate after the timed
         # steps (per-launch events slow a traversal by ~15%, so the timed
         # steps run without them)
-        tparams = flk.default_params(method=args.method, criterion=args.criterion, timing=1)
+        tparams = flk.default_params(method=args.method, criterion=args.criterion, timing=1,
+                                     bh_direct_threshold=args.direct_threshold)
         ms_level = eng.linking_numbers(pairs, tparams).stats["ms_bh_level"]
         gbps = st["bh_visits"] * BH_BYTES_PER_VISIT / (ms_level * 1e-3) * 1e-9
         t = measured.get("bh_level/random_walks_2x8M_beta") if (args.workload, args.scale, args.criterion) == (5, 1.0, "beta") else None
@@ -214,8 +231,12 @@ def main():
             "candidate_pairs": int(len(pairs)),
             "seg_pairs": seg_pairs,
             "linked_pairs": len(cert),
+            "certificate_sha1": hashlib.sha1(repr(cert).encode()).hexdigest(),
             "max_off_integer": float(off_int.max()) if len(off_int) else 0.0,
             "fallbacks": int(st["fallbacks"]),
+            "direct_threshold": args.direct_threshold,
+            "ds_seg_pairs": int(st["ds_seg_pairs"]),
+            "bh_visits": int(st["bh_visits"]),
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -312,11 +312,16 @@ class Engine:
 def certificate_from_lk(pairs: np.ndarray, lk: np.ndarray) -> list[tuple[int, int, int]]:
     """Rounds pair sums into certificate triplets as the reference does
     (src/linking_number_certificate.cpp:185-213, :294-322): nearest integer
-    (half away from zero), kept when |round| >= 0.25, sorted by (i, j)."""
-    out = []
-    for (i, j), v in zip(np.asarray(pairs).reshape(-1, 2), lk):
-        r = np.sign(v) * np.floor(abs(v) + 0.5) if np.isfinite(v) else v
-        if not (abs(r) < 0.25):
-            out.append((int(min(i, j)), int(max(i, j)), int(r) if np.isfinite(r) else -2 ** 31))
-    out.sort()
-    return out
+    (half away from zero), kept when |round| >= 0.25, sorted by (i, j); a
+    non-finite sum is kept with the value -2**31."""
+    p = np.asarray(pairs, dtype=np.int64).reshape(-1, 2)
+    v = np.asarray(lk, dtype=np.float64).reshape(-1)
+    fin = np.isfinite(v)
+    r = np.where(fin, np.sign(v) * np.floor(np.abs(np.where(fin, v, 0.0)) + 0.5), 0.0)
+    keep = ~fin | (np.abs(r) >= 0.25)
+    i = np.minimum(p[keep, 0], p[keep, 1])
+    j = np.maximum(p[keep, 0], p[keep, 1])
+    val = np.where(fin[keep], r[keep], 0.0).astype(np.int64)
+    val[~fin[keep]] = -2 ** 31
+    order = np.lexsort((val, j, i))
+    return list(zip(i[order].tolist(), j[order].tolist(), val[order].tolist()))

@@ -1208,15 +1208,20 @@ void bh_run(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, const flk_params
   if (bh_idx.empty()) return;
   // Batches of pairs whose curves' trees fit the device: the segment budget
   // is bh_batch_ksegs thousand, or what free memory (plus the scratch this
-  // context already holds) allows.  A curve shared by two batches is built
-  // in both.
+  // context already holds) allows when a curve set first runs Barnes–Hut --
+  // kept for later calls on the same curves, so repeated certificates form
+  // the same batches and reuse the same scratch.  A curve shared by two
+  // batches is built in both.
   int64_t budget;
   if (prm.bh_batch_ksegs > 0) {
     budget = (int64_t)prm.bh_batch_ksegs * 1000;
   } else {
-    size_t free_b = 0, total_b = 0;
-    FLK_HIP(hipMemGetInfo(&free_b, &total_b));
-    budget = (int64_t)(0.6 * (double)(free_b + ctx->bh.bytes()) / kBhBytesPerSegment);
+    if (cv.bh_budget <= 0) {
+      size_t free_b = 0, total_b = 0;
+      FLK_HIP(hipMemGetInfo(&free_b, &total_b));
+      cv.bh_budget = (int64_t)(0.6 * (double)(free_b + ctx->bh.bytes()) / kBhBytesPerSegment);
+    }
+    budget = cv.bh_budget;
   }
   budget = std::max<int64_t>(budget, 1);
   std::vector<char> in_batch(cv.ncurves, 0);

@@ -174,6 +174,7 @@ int32_t flk_set_curves(flk_ctx *ctx, const double *xyz, const int64_t *offsets, 
     cv.lo.reserve(sizeof(float4) * std::max<int64_t>(cv.nvert_dev, 1));
     cv.ed.reserve(sizeof(float4) * std::max<int64_t>(cv.nvert_dev, 1));
     cv.inexact = false;
+    cv.bh_budget = 0;
     if (cv.nvert_dev > 0) {
       flk::stage_vertices_launch(ctx);
       ctx->d_count.reserve(sizeof(int32_t));

@@ -121,6 +121,7 @@ struct Curves {
   DevBuf d_off;                  // int64 device offsets (ncurves + 1)
   int64_t nvert_dev = 0;
   bool inexact = false;          // some stored coordinate needs the FP32 lo part
+  int64_t bh_budget = 0;         // Barnes–Hut batch size in segments, fixed per curve set
 };
 
 // Barnes–Hut device scratch, kept by the context between calls (grow-only)

@@ -324,3 +324,16 @@ def test_flk_library_exports_every_header_symbol():
         assert hasattr(lib, n), n
     from fastlinkingnumbers_amd import FLK_EXPORTS
     assert {n for n, _, _ in FLK_EXPORTS} == names
+
+
+def test_certificate_rounding_rules():
+    """Host rounding of pair sums into certificate entries (reference
+    linking_number_certificate.cpp:185-213): nearest integer with halves away
+    from zero, zeros dropped, a non-finite sum kept (as INT_MIN), (min, max)
+    curve order, sorted."""
+    import fastlinkingnumbers_amd as flk
+    pairs = [[4, 1], [0, 2], [0, 3], [5, 2], [1, 2], [3, 5], [2, 3], [0, 1]]
+    lk = [0.49, 0.5, -0.5, 1.5, -2.5, 0.2, float("nan"), 2.9999]
+    assert flk.certificate_from_lk(pairs, lk) == [
+        (0, 1, 3), (0, 2, 1), (0, 3, -1), (1, 2, -3), (2, 3, -2 ** 31), (2, 5, 2)]
+    assert flk.certificate_from_lk(np.zeros((0, 2), np.int32), np.zeros(0)) == []
