@@ -49,7 +49,8 @@ class FlkParams(C.Structure):
         ("mlp_log_tol", C.c_double),
         ("bh_batch_ksegs", C.c_int32),
         ("timing", C.c_int32),
-        ("reserved", C.c_int32 * 6),
+        ("graph", C.c_int32),
+        ("reserved", C.c_int32 * 5),
     ]
 
 
@@ -81,6 +82,20 @@ class FlkStats(C.Structure):
 
 # Every function declared in include/flk.h: (name, restype, argtypes).
 _P = C.POINTER
+class FlkTune(C.Structure):
+    """Kernel launch shapes (include/flk.h flk_tune)."""
+    _fields_ = [
+        ("ds_rows_per_lane", C.c_int32),
+        ("ds_blocks_per_cu", C.c_int32),
+        ("ds_tasks_per_wave", C.c_int32),
+        ("bh_items_per_thread", C.c_int32),
+        ("reserved", C.c_int32 * 4),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
 FLK_EXPORTS = [
     ("flk_default_params", C.c_int32, [_P(FlkParams)]),
     ("flk_create", C.c_int32, [C.c_int32, _P(C.c_void_p)]),
@@ -95,6 +110,9 @@ FLK_EXPORTS = [
      [C.c_void_p, _P(C.c_int32), C.c_int64, _P(FlkParams), _P(C.c_double), _P(C.c_double),
       _P(C.c_uint32)]),
     ("flk_get_stats", C.c_int32, [C.c_void_p, _P(FlkStats)]),
+    ("flk_get_tune", C.c_int32, [C.c_void_p, _P(FlkTune)]),
+    ("flk_set_tune", C.c_int32, [C.c_void_p, _P(FlkTune)]),
+    ("flk_autotune", C.c_int32, [C.c_void_p, _P(C.c_int32), C.c_int64, _P(FlkParams), _P(FlkTune)]),
     ("flk_mlp_set_weights", C.c_int32, [C.c_void_p, _P(C.c_float), _P(C.c_float)]),
     ("flk_mlp_get_weights", C.c_int32, [C.c_void_p, _P(C.c_float), _P(C.c_float)]),
     ("flk_mlp_forward", C.c_int32, [C.c_void_p, _P(C.c_float), C.c_int64, _P(C.c_float)]),
@@ -259,6 +277,30 @@ class Engine:
         s = FlkStats()
         self._check(self.lib.flk_get_stats(self.h, C.byref(s)), "flk_get_stats")
         return s.as_dict()
+
+    # ---- launch shapes
+    def get_tune(self) -> dict:
+        t = FlkTune()
+        self._check(self.lib.flk_get_tune(self.h, C.byref(t)), "flk_get_tune")
+        return t.as_dict()
+
+    def set_tune(self, **kw) -> dict:
+        t = FlkTune()
+        self._check(self.lib.flk_get_tune(self.h, C.byref(t)), "flk_get_tune")
+        for k, v in kw.items():
+            setattr(t, k, v)
+        self._check(self.lib.flk_set_tune(self.h, C.byref(t)), "flk_set_tune")
+        return t.as_dict()
+
+    def autotune(self, pairs: np.ndarray, params: FlkParams | None = None, **kw) -> dict:
+        """Times candidate launch shapes on these pairs and keeps the fastest."""
+        pairs = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1, 2))
+        if params is None:
+            params = default_params(**kw)
+        t = FlkTune()
+        self._check(self.lib.flk_autotune(self.h, _ptr(pairs, C.c_int32), len(pairs), C.byref(params),
+                                          C.byref(t)), "flk_autotune")
+        return t.as_dict()
 
     # ---- learned criterion
     def mlp_set_weights(self, params: np.ndarray, norm: np.ndarray):

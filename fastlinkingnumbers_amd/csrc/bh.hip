@@ -407,14 +407,13 @@ __device__ __forceinline__ int64_t wave_append(unsigned long long *counter, int 
   return (int64_t)base + incl - cnt;
 }
 
-// Traversal blocks: 256 threads, each classifying kItemsPerThread items, so
-// one block owns 1024 consecutive items.  Queue slots, sums and statistics are
-// reduced in LDS and published with one global atomic per block: atomics on a
-// single address serialise across the chip (one per wave capped a 4M-item
-// level at ~7 ns per atomic), and one per 1024 items removes that ceiling.
+// Traversal blocks: 256 threads, each classifying IPT items (2, 4 or 8, a
+// tunable shape), so one block owns 256 * IPT consecutive items.  Queue slots,
+// sums and statistics are reduced in LDS and published with one global atomic
+// per block: atomics on a single address serialise across the chip (one per
+// wave capped a 4M-item level at ~7 ns per atomic), and one per 1024 items
+// removes that ceiling.
 constexpr int kTravBlock = 256;
-constexpr int kItemsPerThread = 4;
-constexpr int kTravItems = kTravBlock * kItemsPerThread;
 
 struct TravShared {
   double v[4], e[4];
@@ -498,8 +497,9 @@ __device__ __forceinline__ int wave_lds_slot(bool pred, int *counter) {
 // One traversal level over a chunk of items.  Split items are staged in LDS
 // as (pair with the split side in its sign, unsplit node, child 0, child 1)
 // from the front of the block's stage, leaf-bucket items from the back; after
-// the block has classified its 1024 items the stage is copied out coalesced
-// behind one queue atomic each.
+// the block has classified its 256 * IPT items the stage is copied out
+// coalesced behind one queue atomic each.
+template <int IPT>
 __global__ __launch_bounds__(kTravBlock) void bh_level(
     const BhItem *__restrict__ items, int64_t n, const BhGeo *__restrict__ geo,
     const BhMom *__restrict__ mom, const float *__restrict__ beta2, float eps,
@@ -507,6 +507,7 @@ __global__ __launch_bounds__(kTravBlock) void bh_level(
     BhItem *__restrict__ next, unsigned long long *__restrict__ next_count, int64_t next_cap,
     BhItem *__restrict__ leaves, unsigned long long *__restrict__ leaf_count, int64_t leaf_cap,
     double *__restrict__ psum, double *__restrict__ perr) {
+  constexpr int kTravItems = kTravBlock * IPT;
   __shared__ int4 stage[kTravItems];
   __shared__ TravShared sh;
   if (threadIdx.x == 0) sh.nsplit = sh.nleaf = 0;
@@ -515,7 +516,7 @@ __global__ __launch_bounds__(kTravBlock) void bh_level(
   int tp = -1;  // this thread's far-field pair and its running sums
   double tv = 0.0, te = 0.0;
 #pragma unroll 1
-  for (int k = 0; k < kItemsPerThread; ++k) {
+  for (int k = 0; k < IPT; ++k) {
     const int64_t t = b0 + (int64_t)k * kTravBlock;
     const bool live = t < n;
     BhItem it = live ? items[t] : BhItem{0, 0, 0, 0};
@@ -595,19 +596,21 @@ __global__ __launch_bounds__(kTravBlock) void bh_level(
 
 // Exact evaluation of leaf-bucket pairs: every segment pair of the two
 // buckets with the reference leaf term, from FP32 hi/lo vertices.  Blocks of
-// 1024 items as in bh_level; counters[2] gets the segment-pair count.
+// 256 * IPT items as in bh_level; counters[2] gets the segment-pair count.
+template <int IPT>
 __global__ __launch_bounds__(kTravBlock) void bh_leaf_pairs(
     const BhItem *__restrict__ items, int64_t n, const BhGeo *__restrict__ geo,
     const uint32_t *__restrict__ order, const float4 *__restrict__ hi,
     const float4 *__restrict__ lo, const float4 *__restrict__ ed, double *__restrict__ psum,
     double *__restrict__ perr, unsigned long long *__restrict__ counters) {
+  constexpr int kTravItems = kTravBlock * IPT;
   __shared__ TravShared sh;
   const int64_t b0 = (int64_t)blockIdx.x * kTravItems + threadIdx.x;
   int tp = -1;
   double tv = 0.0;
   int npairs = 0;
 #pragma unroll 1
-  for (int k = 0; k < kItemsPerThread; ++k) {
+  for (int k = 0; k < IPT; ++k) {
     const int64_t t = b0 + (int64_t)k * kTravBlock;
     if (t >= n) break;
     const BhItem it = items[t];
@@ -896,6 +899,10 @@ void traverse(flk_ctx *ctx, Forest &F, const std::vector<int32_t> &ra, const std
   if (np == 0) return;
   constexpr int64_t kChunk = 1 << 24;
   constexpr int kSlots = 1024;  // count pairs per zeroing of the counter array
+  // items per thread of the level/leaf kernels: 2, 4 or 8 (grid and kernel agree)
+  const int ipt = (ctx->tune.bh_items_per_thread == 2 || ctx->tune.bh_items_per_thread == 8)
+                      ? ctx->tune.bh_items_per_thread
+                      : 4;
   B.roots_a.reserve(sizeof(int32_t) * np);
   B.roots_b.reserve(sizeof(int32_t) * np);
   B.beta2.reserve(sizeof(float) * np);
@@ -954,11 +961,14 @@ void traverse(flk_ctx *ctx, Forest &F, const std::vector<int32_t> &ra, const std
                         timing ? ctx->ev3 : nullptr);
     unsigned long long *qc = cnt + 4 + 2 * slot;  // [0] next items, [1] leaf items
     // with timing, start/stop events ride on the dispatch itself
-    launch_timed(timing ? ctx->ev1 : nullptr, timing ? ctx->ev2 : nullptr, bh_level,
-                 dim3((unsigned)((take + kTravItems - 1) / kTravItems)), dim3(kTravBlock), s, src, take,
-                 F.geo.as<BhGeo>(), F.mom.as<BhMom>(), B.beta2.as<float>(), eps,
-                 use_mlp ? B.pred.as<float>() : nullptr, log_tol, spare->as<BhItem>(), qc, 2 * take,
-                 B.leaves.as<BhItem>(), qc + 1, take, d_sum, d_err);
+    {
+      const dim3 grid((unsigned)((take + ipt * kTravBlock - 1) / (ipt * kTravBlock)));
+      auto lev = ipt == 2 ? bh_level<2> : ipt == 8 ? bh_level<8> : bh_level<4>;
+      launch_timed(timing ? ctx->ev1 : nullptr, timing ? ctx->ev2 : nullptr, lev, grid, dim3(kTravBlock), s,
+                   src, take, F.geo.as<BhGeo>(), F.mom.as<BhMom>(), (const float *)B.beta2.as<float>(), eps,
+                   (const float *)(use_mlp ? B.pred.as<float>() : nullptr), log_tol, spare->as<BhItem>(), qc,
+                   2 * take, B.leaves.as<BhItem>(), qc + 1, take, d_sum, d_err);
+    }
     FLK_HIP(hipGetLastError());
     FLK_HIP(hipMemcpyAsync(B.host_cnt, qc, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     FLK_HIP(hipStreamSynchronize(s));
@@ -978,7 +988,8 @@ void traverse(flk_ctx *ctx, Forest &F, const std::vector<int32_t> &ra, const std
     far += (unsigned long long)take - nn[0] / 2 - nn[1];
     if (nn[1] > 0) {
       const Curves &cv = ctx->curves;
-      hipLaunchKernelGGL(bh_leaf_pairs, dim3((unsigned)((nn[1] + kTravItems - 1) / kTravItems)),
+      auto leafk = ipt == 2 ? bh_leaf_pairs<2> : ipt == 8 ? bh_leaf_pairs<8> : bh_leaf_pairs<4>;
+      hipLaunchKernelGGL(leafk, dim3((unsigned)((nn[1] + ipt * kTravBlock - 1) / (ipt * kTravBlock))),
                          dim3(kTravBlock), 0, s, B.leaves.as<BhItem>(), (int64_t)nn[1], F.geo.as<BhGeo>(),
                          F.order.as<uint32_t>(), cv.hi.as<float4>(), cv.lo.as<float4>(),
                          cv.ed.as<float4>(), d_sum, d_err, cnt);
@@ -1201,7 +1212,7 @@ void bh_run(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, const flk_params
   if (!ds_idx.empty()) {
     std::vector<double> v;
     float ms = 0.f;
-    ds_run(ctx, ds_pairs.data(), (int64_t)ds_idx.size(), prm.fp64 != 0, v, &ms);
+    ds_run(ctx, ds_pairs.data(), (int64_t)ds_idx.size(), prm.fp64 != 0, v, &ms, prm.graph != 0);
     ctx->stats.ms_ds_kernel += ms;
     for (size_t t = 0; t < ds_idx.size(); ++t) lk[ds_idx[t]] = v[t];
   }

@@ -386,103 +386,281 @@ void convert_vertices_launch(hipStream_t s, const double4 *d, int64_t n, float4 
 constexpr int kRowsPerLane = 8;
 constexpr int kRowsPerLane64 = 2;
 
-void ds_run(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, bool fp64,
-            std::vector<double> &lk_out, float *ms_kernel) {
-  Curves &cv = ctx->curves;
-  lk_out.assign(npairs, 0.0);
-  if (ms_kernel) *ms_kernel = 0.f;
-  if (npairs == 0) return;
-  const int R = fp64 ? kRowsPerLane64 : kRowsPerLane;
-  const int rows_per_wave = 64 * R;
-  // Persistent grid: 4 blocks of 4 waves per CU.
-  const int blocks = std::max(1, ctx->num_cus) * 4;
-  const int64_t nwaves = (int64_t)blocks * 4;
+namespace {
 
-  std::vector<DsPair> dp(npairs);
-  int64_t total_pairs = 0, wave_cost = 0;
+// Blocks of 256 threads per CU at which every block of `kernel` is resident
+// (the persistent grid gives each wave an equal share of the work, so blocks
+// that wait for a slot would run as a second, under-occupied round).
+int resident_blocks(const void *kernel) {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, 256, 0) != hipSuccess || nb < 1) nb = 1;
+  return nb;
+}
+
+// Host plan of one batched direct sum: pair records (rows = longer curve),
+// the tasks of every `stride`-th block of 64R rows (stride 1: all of them),
+// their cost prefix and each pair's first task.
+struct DsPlan {
+  int R = 8, blocks = 0;
+  bool fp64 = false, precise = false;
+  int64_t work = 0;
+  std::vector<DsPair> dp;
+  std::vector<DsTask> tasks;
+  std::vector<int64_t> prefix, task0;
+};
+
+void ds_plan(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, bool fp64, const flk_tune &t, int64_t stride,
+             DsPlan &P) {
+  Curves &cv = ctx->curves;
+  P.fp64 = fp64;
+  P.precise = cv.inexact;
+  P.R = fp64 ? kRowsPerLane64 : (t.ds_rows_per_lane == 4 ? 4 : 8);
+  const int rows_per_wave = 64 * P.R;
+  int bpc = t.ds_blocks_per_cu;
+  if (bpc <= 0) {
+    static int cached[5] = {0, 0, 0, 0, 0};  // f64, f32 R4 exact/precise, R8 exact/precise
+    const int slot = fp64 ? 0 : 1 + (P.R == 8 ? 2 : 0) + (P.precise ? 1 : 0);
+    if (!cached[slot]) {
+      const void *k = fp64 ? (const void *)ds_tile_f64<kRowsPerLane64>
+                    : P.R == 4 ? (P.precise ? (const void *)ds_tile_f32<4, true> : (const void *)ds_tile_f32<4, false>)
+                               : (P.precise ? (const void *)ds_tile_f32<8, true> : (const void *)ds_tile_f32<8, false>);
+      cached[slot] = resident_blocks(k);
+    }
+    bpc = cached[slot];
+  }
+  P.blocks = std::max(1, ctx->num_cus) * bpc;
+  const int64_t nwaves = (int64_t)P.blocks * 4;
+
+  P.dp.assign(npairs, DsPair{});
+  std::vector<std::pair<int32_t, int32_t>> rowblk;  // (pair, first row) of the blocks run
+  int64_t wave_cost = 0, blk = 0;
+  P.work = 0;
   for (int64_t k = 0; k < npairs; ++k) {
     int a = pairs[2 * k], b = pairs[2 * k + 1];
     if (cv.n[a] < cv.n[b]) std::swap(a, b);  // rows = longer curve
-    dp[k] = DsPair{cv.off[a], cv.off[b], cv.n[a], cv.n[b]};
-    total_pairs += (int64_t)cv.n[a] * cv.n[b];
-    wave_cost += (int64_t)((cv.n[a] + rows_per_wave - 1) / rows_per_wave) * rows_per_wave * cv.n[b];
+    P.dp[k] = DsPair{cv.off[a], cv.off[b], cv.n[a], cv.n[b]};
+    if (cv.n[a] == 0 || cv.n[b] == 0) continue;
+    for (int32_t r0 = 0; r0 < cv.n[a]; r0 += rows_per_wave, ++blk) {
+      if (blk % stride) continue;
+      rowblk.emplace_back((int32_t)k, r0);
+      wave_cost += (int64_t)rows_per_wave * cv.n[b];
+      P.work += (int64_t)std::min(rows_per_wave, cv.n[a] - r0) * cv.n[b];
+    }
   }
-  // Column chunk: aim for >= 8 tasks per wave over the whole batch, never
-  // below 256 columns.
-  const int64_t target_task = std::max<int64_t>(wave_cost / (nwaves * 8), (int64_t)rows_per_wave * 256);
+  // Column chunk: aim for ds_tasks_per_wave tasks per wave over the batch,
+  // never below 256 columns.
+  const int64_t tpw = std::max(1, t.ds_tasks_per_wave);
+  const int64_t target_task = std::max<int64_t>(wave_cost / (nwaves * tpw), (int64_t)rows_per_wave * 256);
   const int64_t chunk_cols = std::max<int64_t>(256, target_task / rows_per_wave);
-
-  std::vector<DsTask> tasks;
-  std::vector<int64_t> prefix, task0(npairs + 1);
-  tasks.reserve(npairs * 4);
-  prefix.reserve(npairs * 4 + 1);
+  P.tasks.clear();
+  P.prefix.clear();
+  P.task0.assign(npairs + 1, 0);
+  P.tasks.reserve(rowblk.size() * 2);
+  P.prefix.reserve(rowblk.size() * 2 + 1);
   int64_t acc = 0;
+  size_t rb = 0;
   for (int64_t k = 0; k < npairs; ++k) {
-    task0[k] = (int64_t)tasks.size();
-    const DsPair &p = dp[k];
-    if (p.n_r == 0 || p.n_c == 0) continue;
-    for (int32_t r0 = 0; r0 < p.n_r; r0 += rows_per_wave) {
+    P.task0[k] = (int64_t)P.tasks.size();
+    for (; rb < rowblk.size() && rowblk[rb].first == k; ++rb) {
+      const DsPair &p = P.dp[k];
       for (int64_t c0 = 0; c0 < p.n_c; c0 += chunk_cols) {
         const int32_t nc = (int32_t)std::min<int64_t>(chunk_cols, p.n_c - c0);
-        tasks.push_back(DsTask{(int32_t)k, r0, (int32_t)c0, nc});
-        prefix.push_back(acc);
+        P.tasks.push_back(DsTask{(int32_t)k, rowblk[rb].second, (int32_t)c0, nc});
+        P.prefix.push_back(acc);
         acc += (int64_t)nc * rows_per_wave;
       }
     }
   }
-  task0[npairs] = (int64_t)tasks.size();
-  prefix.push_back(acc);
-  const int64_t ntasks = (int64_t)tasks.size();
-  ctx->stats.ds_seg_pairs += total_pairs;
-  if (ntasks == 0) return;
+  P.task0[npairs] = (int64_t)P.tasks.size();
+  P.prefix.push_back(acc);
+}
 
-  ctx->d_pairs.reserve(sizeof(DsPair) * npairs);
-  ctx->d_tasks.reserve(sizeof(DsTask) * ntasks);
-  ctx->d_cost.reserve(sizeof(int64_t) * (ntasks + 1));
-  ctx->d_task_theta.reserve(sizeof(double) * ntasks);
-  ctx->d_task_corr.reserve(sizeof(int32_t) * ntasks);
-  ctx->d_pair_task0.reserve(sizeof(int64_t) * (npairs + 1));
-  ctx->d_pair_out.reserve(sizeof(double) * npairs);
-  hipStream_t s = ctx->stream;
-  FLK_HIP(hipMemcpyAsync(ctx->d_pairs.ptr, dp.data(), sizeof(DsPair) * npairs, hipMemcpyHostToDevice, s));
-  FLK_HIP(hipMemcpyAsync(ctx->d_tasks.ptr, tasks.data(), sizeof(DsTask) * ntasks, hipMemcpyHostToDevice, s));
-  FLK_HIP(hipMemcpyAsync(ctx->d_cost.ptr, prefix.data(), sizeof(int64_t) * (ntasks + 1), hipMemcpyHostToDevice, s));
-  FLK_HIP(hipMemcpyAsync(ctx->d_pair_task0.ptr, task0.data(), sizeof(int64_t) * (npairs + 1), hipMemcpyHostToDevice, s));
+// Device buffers of one batch.
+struct DsBufs {
+  DsPair *pairs;
+  DsTask *tasks;
+  int64_t *cost, *task0;
+  double *theta, *out;
+  int32_t *corr;
+};
 
-  FLK_HIP(hipEventRecord(ctx->ev0, s));
-  if (fp64) {
-    hipLaunchKernelGGL((ds_tile_f64<kRowsPerLane64>), dim3(blocks), dim3(256), 0, s,
-                       cv.dpts.as<double4>(), ctx->d_pairs.as<DsPair>(), ctx->d_tasks.as<DsTask>(),
-                       ctx->d_cost.as<int64_t>(), ntasks, ctx->d_task_theta.as<double>(),
-                       ctx->d_task_corr.as<int32_t>());
-  } else if (cv.inexact) {
-    hipLaunchKernelGGL((ds_tile_f32<kRowsPerLane, true>), dim3(blocks), dim3(256), 0, s,
-                       cv.hi.as<float4>(), cv.lo.as<float4>(), cv.ed.as<float4>(),
-                       ctx->d_pairs.as<DsPair>(), ctx->d_tasks.as<DsTask>(),
-                       ctx->d_cost.as<int64_t>(), ntasks, ctx->d_task_theta.as<double>(),
-                       ctx->d_task_corr.as<int32_t>());
+DsBufs reserve_bufs(DevBuf &pairs, DevBuf &tasks, DevBuf &cost, DevBuf &theta, DevBuf &corr, DevBuf &task0,
+                    DevBuf &out, size_t np, size_t nt) {
+  pairs.reserve(sizeof(DsPair) * np);
+  tasks.reserve(sizeof(DsTask) * nt);
+  cost.reserve(sizeof(int64_t) * (nt + 1));
+  theta.reserve(sizeof(double) * nt);
+  corr.reserve(sizeof(int32_t) * nt);
+  task0.reserve(sizeof(int64_t) * (np + 1));
+  out.reserve(sizeof(double) * np);
+  return DsBufs{pairs.as<DsPair>(), tasks.as<DsTask>(), cost.as<int64_t>(), task0.as<int64_t>(),
+                theta.as<double>(), out.as<double>(), corr.as<int32_t>()};
+}
+
+// Uploads of the plan arrays, the tile kernel (between ev_a and ev_b when
+// given) and, with `out`, the per-pair reduction and read-back: one stream-
+// ordered sequence, launched eagerly or captured into a graph.
+void ds_enqueue(flk_ctx *ctx, const DsPlan &P, const DsPair *dp, const DsTask *tasks, const int64_t *prefix,
+                const int64_t *task0, int64_t npairs, int64_t ntasks, const DsBufs &d, hipStream_t s,
+                hipEvent_t ev_a, hipEvent_t ev_b, double *out) {
+  Curves &cv = ctx->curves;
+  FLK_HIP(hipMemcpyAsync(d.pairs, dp, sizeof(DsPair) * npairs, hipMemcpyHostToDevice, s));
+  FLK_HIP(hipMemcpyAsync(d.tasks, tasks, sizeof(DsTask) * ntasks, hipMemcpyHostToDevice, s));
+  FLK_HIP(hipMemcpyAsync(d.cost, prefix, sizeof(int64_t) * (ntasks + 1), hipMemcpyHostToDevice, s));
+  FLK_HIP(hipMemcpyAsync(d.task0, task0, sizeof(int64_t) * (npairs + 1), hipMemcpyHostToDevice, s));
+  if (ev_a) FLK_HIP(hipEventRecord(ev_a, s));
+  const float4 *hi = cv.hi.as<float4>(), *lo = cv.lo.as<float4>(), *ed = cv.ed.as<float4>();
+  const dim3 grid(P.blocks), block(256);
+  if (P.fp64) {
+    hipLaunchKernelGGL((ds_tile_f64<kRowsPerLane64>), grid, block, 0, s, cv.dpts.as<double4>(), d.pairs, d.tasks,
+                       d.cost, ntasks, d.theta, d.corr);
+  } else if (P.R == 4) {
+    if (P.precise)
+      hipLaunchKernelGGL((ds_tile_f32<4, true>), grid, block, 0, s, hi, lo, ed, d.pairs, d.tasks, d.cost, ntasks,
+                         d.theta, d.corr);
+    else
+      hipLaunchKernelGGL((ds_tile_f32<4, false>), grid, block, 0, s, hi, lo, ed, d.pairs, d.tasks, d.cost, ntasks,
+                         d.theta, d.corr);
   } else {
-    hipLaunchKernelGGL((ds_tile_f32<kRowsPerLane, false>), dim3(blocks), dim3(256), 0, s,
-                       cv.hi.as<float4>(), cv.lo.as<float4>(), cv.ed.as<float4>(),
-                       ctx->d_pairs.as<DsPair>(), ctx->d_tasks.as<DsTask>(),
-                       ctx->d_cost.as<int64_t>(), ntasks, ctx->d_task_theta.as<double>(),
-                       ctx->d_task_corr.as<int32_t>());
+    if (P.precise)
+      hipLaunchKernelGGL((ds_tile_f32<8, true>), grid, block, 0, s, hi, lo, ed, d.pairs, d.tasks, d.cost, ntasks,
+                         d.theta, d.corr);
+    else
+      hipLaunchKernelGGL((ds_tile_f32<8, false>), grid, block, 0, s, hi, lo, ed, d.pairs, d.tasks, d.cost, ntasks,
+                         d.theta, d.corr);
   }
   FLK_HIP(hipGetLastError());
-  FLK_HIP(hipEventRecord(ctx->ev1, s));
-  const int64_t rblocks = (npairs * 64 + 255) / 256;
-  hipLaunchKernelGGL(ds_pair_reduce, dim3((unsigned)rblocks), dim3(256), 0, s,
-                     ctx->d_task_theta.as<double>(), ctx->d_task_corr.as<int32_t>(),
-                     ctx->d_pair_task0.as<int64_t>(), npairs, ctx->d_pair_out.as<double>());
-  FLK_HIP(hipGetLastError());
-  FLK_HIP(hipMemcpyAsync(lk_out.data(), ctx->d_pair_out.ptr, sizeof(double) * npairs,
-                         hipMemcpyDeviceToHost, s));
+  if (ev_b) FLK_HIP(hipEventRecord(ev_b, s));
+  if (out) {
+    const int64_t rblocks = (npairs * 64 + 255) / 256;
+    hipLaunchKernelGGL(ds_pair_reduce, dim3((unsigned)rblocks), dim3(256), 0, s, d.theta, d.corr, d.task0,
+                       npairs, d.out);
+    FLK_HIP(hipGetLastError());
+    FLK_HIP(hipMemcpyAsync(out, d.out, sizeof(double) * npairs, hipMemcpyDeviceToHost, s));
+  }
+}
+
+// Eager run of a plan on the context's shared buffers; returns the tile
+// kernel's device time (ms).
+float ds_eager(flk_ctx *ctx, const DsPlan &P, std::vector<double> *lk_out) {
+  if (P.tasks.empty()) return 0.f;
+  const int64_t np = (int64_t)P.dp.size(), nt = (int64_t)P.tasks.size();
+  const DsBufs d = reserve_bufs(ctx->d_pairs, ctx->d_tasks, ctx->d_cost, ctx->d_task_theta, ctx->d_task_corr,
+                                ctx->d_pair_task0, ctx->d_pair_out, np, nt);
+  hipStream_t s = ctx->stream;
+  ds_enqueue(ctx, P, P.dp.data(), P.tasks.data(), P.prefix.data(), P.task0.data(), np, nt, d, s, ctx->ev0,
+             ctx->ev1, lk_out ? lk_out->data() : nullptr);
   FLK_HIP(hipStreamSynchronize(s));
   float ms = 0.f;
   FLK_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  return ms;
+}
+
+uint64_t fnv1a(uint64_t h, const void *p, size_t n) {
+  const unsigned char *c = static_cast<const unsigned char *>(p);
+  for (size_t i = 0; i < n; ++i) h = (h ^ c[i]) * 0x100000001b3ULL;
+  return h;
+}
+
+// Graph run: the batch is captured once per (pairs, shape, precision, curve
+// set) and replayed after that.  The graph is timed whole (its uploads and
+// the reduction add well under a percent to a batch worth a launch).
+float ds_graph_run(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, bool fp64, std::vector<double> &lk_out,
+                   int64_t *work) {
+  const flk_tune &t = ctx->tune;
+  uint64_t key = fnv1a(0xcbf29ce484222325ULL, pairs, sizeof(int32_t) * 2 * (size_t)npairs);
+  const int64_t meta[7] = {npairs, t.ds_rows_per_lane, t.ds_blocks_per_cu, t.ds_tasks_per_wave, fp64 ? 1 : 0,
+                           (int64_t)ctx->curves_version, ctx->curves.inexact ? 1 : 0};
+  key = fnv1a(key, meta, sizeof meta);
+  DsGraph &G = ctx->ds_graph[fp64 ? 1 : 0];
+  hipStream_t s = ctx->stream;
+  if (!(G.exec && G.key == key)) {
+    G.reset();
+    DsPlan P;
+    ds_plan(ctx, pairs, npairs, fp64, t, 1, P);
+    if (P.tasks.empty()) {
+      lk_out.assign(npairs, 0.0);
+      *work = 0;
+      return 0.f;
+    }
+    // the graph's copy nodes read the host arrays kept in G
+    G.dp = std::move(P.dp);
+    G.tasks = std::move(P.tasks);
+    G.prefix = std::move(P.prefix);
+    G.task0 = std::move(P.task0);
+    G.out.assign(npairs, 0.0);
+    G.work = P.work;
+    G.waves = P.blocks * 4;
+    const int64_t nt = (int64_t)G.tasks.size();
+    const DsBufs d =
+        reserve_bufs(G.d_pairs, G.d_tasks, G.d_cost, G.d_theta, G.d_corr, G.d_task0, G.d_out, npairs, nt);
+    FLK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipGraph_t graph = nullptr;
+    try {
+      ds_enqueue(ctx, P, G.dp.data(), G.tasks.data(), G.prefix.data(), G.task0.data(), npairs, nt, d, s, nullptr,
+                 nullptr, G.out.data());
+    } catch (...) {
+      (void)hipStreamEndCapture(s, &graph);
+      if (graph) (void)hipGraphDestroy(graph);
+      throw;
+    }
+    FLK_HIP(hipStreamEndCapture(s, &graph));
+    const hipError_t ie = hipGraphInstantiate(&G.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) {
+      G.exec = nullptr;
+      throw Error(FLK_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+    }
+    G.key = key;
+  }
+  FLK_HIP(hipEventRecord(ctx->ev0, s));
+  FLK_HIP(hipGraphLaunch(G.exec, s));
+  FLK_HIP(hipEventRecord(ctx->ev1, s));
+  FLK_HIP(hipStreamSynchronize(s));
+  float ms = 0.f;
+  FLK_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  lk_out = G.out;
+  *work = G.work;
+  ctx->stats.ds_waves = G.waves;
+  return ms;
+}
+
+}  // namespace
+
+void ds_run(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, bool fp64,
+            std::vector<double> &lk_out, float *ms_kernel, bool use_graph) {
+  lk_out.assign(npairs, 0.0);
+  float ms = 0.f;
+  int64_t work = 0;
+  if (npairs > 0) {
+    if (use_graph) {
+      ms = ds_graph_run(ctx, pairs, npairs, fp64, lk_out, &work);
+    } else {
+      DsPlan P;
+      ds_plan(ctx, pairs, npairs, fp64, ctx->tune, 1, P);
+      work = P.work;
+      ms = ds_eager(ctx, P, &lk_out);
+      ctx->stats.ds_waves = P.blocks * 4;
+    }
+    ctx->stats.ds_launches += 1;
+  }
   if (ms_kernel) *ms_kernel = ms;
-  ctx->stats.ds_launches += 1;
-  ctx->stats.ds_waves = (int32_t)nwaves;
+  ctx->stats.ds_seg_pairs += work;
+}
+
+float ds_time_sample(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, const flk_tune &t, int64_t stride,
+                     int64_t *work) {
+  DsPlan P;
+  ds_plan(ctx, pairs, npairs, false, t, std::max<int64_t>(1, stride), P);
+  if (work) *work = P.work;
+  return ds_eager(ctx, P, nullptr);
+}
+
+int64_t ds_work(flk_ctx *ctx, const int32_t *pairs, int64_t npairs) {
+  const Curves &cv = ctx->curves;
+  int64_t w = 0;
+  for (int64_t k = 0; k < npairs; ++k) w += (int64_t)cv.n[pairs[2 * k]] * cv.n[pairs[2 * k + 1]];
+  return w;
 }
 
 }  // namespace flk

@@ -49,6 +49,7 @@ int32_t flk_default_params(flk_params *p) {
   p->leaf_size = 1;
   p->fp64 = 0;
   p->mlp_log_tol = -6.0;
+  p->graph = 1;
   return FLK_OK;
 }
 
@@ -175,6 +176,8 @@ int32_t flk_set_curves(flk_ctx *ctx, const double *xyz, const int64_t *offsets, 
     cv.ed.reserve(sizeof(float4) * std::max<int64_t>(cv.nvert_dev, 1));
     cv.inexact = false;
     cv.bh_budget = 0;
+    ++ctx->curves_version;
+    for (auto &g : ctx->ds_graph) g.reset();
     if (cv.nvert_dev > 0) {
       flk::stage_vertices_launch(ctx);
       ctx->d_count.reserve(sizeof(int32_t));
@@ -237,7 +240,7 @@ int32_t flk_linking_numbers(flk_ctx *ctx, const int32_t *pairs, int64_t npairs,
     std::vector<uint32_t> f(npairs, 0u);
     if (prm.method == FLK_DIRECTSUM) {
       float ms = 0.f;
-      flk::ds_run(ctx, pairs, npairs, prm.fp64 != 0, v, &ms);
+      flk::ds_run(ctx, pairs, npairs, prm.fp64 != 0, v, &ms, prm.graph != 0);
       ctx->stats.ms_kernel = ms;
       ctx->stats.ms_ds_kernel = ms;
     } else if (prm.method == FLK_BARNESHUT) {
@@ -262,7 +265,7 @@ int32_t flk_linking_numbers(flk_ctx *ctx, const int32_t *pairs, int64_t npairs,
       if (!idx.empty()) {
         std::vector<double> w;
         float ms = 0.f;
-        flk::ds_run(ctx, redo.data(), (int64_t)idx.size(), true, w, &ms);
+        flk::ds_run(ctx, redo.data(), (int64_t)idx.size(), true, w, &ms, prm.graph != 0);
         ctx->stats.ms_fallback = ms;
         ctx->stats.fallbacks = (int64_t)idx.size();
         for (size_t t = 0; t < idx.size(); ++t) {
@@ -286,6 +289,113 @@ int32_t flk_linking_numbers(flk_ctx *ctx, const int32_t *pairs, int64_t npairs,
 int32_t flk_get_stats(const flk_ctx *ctx, flk_stats *out) {
   if (!ctx || !out) return FLK_ERR_ARG;
   *out = ctx->stats;
+  return FLK_OK;
+}
+
+int32_t flk_get_tune(const flk_ctx *ctx, flk_tune *out) {
+  if (!ctx || !out) return FLK_ERR_ARG;
+  *out = ctx->tune;
+  return FLK_OK;
+}
+
+int32_t flk_set_tune(flk_ctx *ctx, const flk_tune *tune) {
+  if (!ctx || !tune) return FLK_ERR_ARG;
+  FLK_GUARD(ctx, {
+    if (tune->ds_rows_per_lane != 4 && tune->ds_rows_per_lane != 8)
+      throw Error(FLK_ERR_ARG, "ds_rows_per_lane must be 4 or 8");
+    if (tune->ds_blocks_per_cu < 0 || tune->ds_blocks_per_cu > 64)
+      throw Error(FLK_ERR_ARG, "ds_blocks_per_cu must be in [0, 64]");
+    if (tune->ds_tasks_per_wave < 1 || tune->ds_tasks_per_wave > 1024)
+      throw Error(FLK_ERR_ARG, "ds_tasks_per_wave must be in [1, 1024]");
+    if (tune->bh_items_per_thread != 2 && tune->bh_items_per_thread != 4 && tune->bh_items_per_thread != 8)
+      throw Error(FLK_ERR_ARG, "bh_items_per_thread must be 2, 4 or 8");
+    ctx->tune = *tune;
+  })
+  return FLK_OK;
+}
+
+// Direct-sum candidates run on every k-th 64R-row block of the pairs' direct-
+// sum work, k chosen so one run covers about a tenth of it (at least 2e10
+// segment pairs, or all of it: smaller samples over-weight the end-of-launch
+// tail); Barnes–Hut candidates run the Barnes–Hut pairs whole.  Each
+// candidate runs twice and keeps its faster time.
+int32_t flk_autotune(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, const flk_params *params,
+                     flk_tune *out) {
+  if (!ctx || (npairs > 0 && !pairs) || npairs < 0) return FLK_ERR_ARG;
+  FLK_GUARD(ctx, {
+    FLK_HIP(hipSetDevice(ctx->device));
+    flk_params prm;
+    if (params) prm = *params;
+    else flk_default_params(&prm);
+    const flk::Curves &cv = ctx->curves;
+    for (int64_t k = 0; k < 2 * npairs; ++k)
+      if (pairs[k] < 0 || pairs[k] >= cv.ncurves) throw Error(FLK_ERR_ARG, "pair index out of range");
+    // split as flk_linking_numbers would
+    std::vector<int32_t> ds_pairs, bh_pairs;
+    for (int64_t k = 0; k < npairs; ++k) {
+      const int32_t a = pairs[2 * k], b = pairs[2 * k + 1];
+      const bool direct = prm.method == FLK_DIRECTSUM || cv.n[a] < 2 || cv.n[b] < 2 ||
+                          (int64_t)cv.n[a] * cv.n[b] < prm.bh_direct_threshold;
+      (direct ? ds_pairs : bh_pairs).insert((direct ? ds_pairs : bh_pairs).end(), {a, b});
+    }
+    const flk_stats saved = ctx->stats;
+    flk_tune best = ctx->tune;
+    const int64_t nds = (int64_t)ds_pairs.size() / 2;
+    if (nds > 0 && !(prm.method == FLK_DIRECTSUM && prm.fp64)) {
+      const int64_t total = flk::ds_work(ctx, ds_pairs.data(), nds);
+      const int64_t target = std::max<int64_t>((int64_t)2e10, total / 10);
+      double best_rate = -1.0;
+      // every stride-th row block: about target / total of the work
+      const int64_t stride = std::max<int64_t>(1, total / target);
+      for (int R : {4, 8}) {
+        for (int bpc : {0, 4}) {  // all blocks resident, or 4 per CU
+          for (int tpw : {8, 16}) {
+            flk_tune t = ctx->tune;
+            t.ds_rows_per_lane = R;
+            t.ds_tasks_per_wave = tpw;
+            t.ds_blocks_per_cu = bpc;
+            float ms = 1e30f;
+            int64_t work = 0;
+            for (int rep = 0; rep < 2; ++rep)
+              ms = std::min(ms, flk::ds_time_sample(ctx, ds_pairs.data(), nds, t, stride, &work));
+            const double rate = ms > 0.f ? (double)work / ms : 0.0;
+            if (rate > best_rate) {
+              best_rate = rate;
+              best.ds_rows_per_lane = t.ds_rows_per_lane;
+              best.ds_blocks_per_cu = t.ds_blocks_per_cu;
+              best.ds_tasks_per_wave = t.ds_tasks_per_wave;
+            }
+          }
+        }
+      }
+    }
+    const int64_t nbh = (int64_t)bh_pairs.size() / 2;
+    if (nbh > 0) {
+      double best_ms = 1e300;
+      flk_params bp = prm;
+      bp.method = FLK_BARNESHUT;
+      bp.bh_direct_threshold = 0;
+      bp.timing = 0;
+      for (int ipt : {2, 4, 8}) {
+        ctx->tune.bh_items_per_thread = ipt;
+        double ms = 1e300;
+        for (int rep = 0; rep < 2; ++rep) {
+          std::vector<double> v, e;
+          std::vector<uint32_t> f;
+          const auto t0 = std::chrono::steady_clock::now();
+          flk::bh_run(ctx, bh_pairs.data(), nbh, bp, v, e, f);
+          ms = std::min(ms, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+        }
+        if (ms < best_ms) {
+          best_ms = ms;
+          best.bh_items_per_thread = ipt;
+        }
+      }
+    }
+    ctx->tune = best;
+    ctx->stats = saved;
+    if (out) *out = best;
+  })
   return FLK_OK;
 }
 

@@ -154,6 +154,28 @@ struct BhScratch {
   }
 };
 
+// A captured direct-sum batch (uploads, tile kernel, per-pair reduction,
+// read-back) for replay when the same pairs come again with the same shape
+// and curves.  It owns its device buffers and host staging, so reallocations
+// elsewhere cannot leave dangling pointers in the graph.
+struct DsGraph {
+  uint64_t key = 0;
+  hipGraphExec_t exec = nullptr;
+  int64_t work = 0;
+  int32_t waves = 0;
+  std::vector<DsPair> dp;
+  std::vector<DsTask> tasks;
+  std::vector<int64_t> prefix, task0;
+  std::vector<double> out;
+  DevBuf d_pairs, d_tasks, d_cost, d_theta, d_corr, d_task0, d_out;
+  void reset() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    exec = nullptr;
+    key = 0;
+  }
+  ~DsGraph() { reset(); }
+};
+
 struct MlpState {
   std::vector<float> params;  // FLK_MLP_NPARAM FP32 masters
   std::vector<float> norm;    // mean[12], std[12]
@@ -175,6 +197,11 @@ struct flk_ctx {
   std::vector<int32_t> pls_pairs;
   flk::MlpState mlp;
   flk::BhScratch bh;
+  // default shapes: measured best on the 1,600-loop chain mail (R = 4 with
+  // every block resident: 985 ms against 1,016 ms for R = 8)
+  flk_tune tune{4, 0, 8, 4, {0, 0, 0, 0}};
+  flk::DsGraph ds_graph[2];      // FP32, FP64 batches
+  uint64_t curves_version = 0;   // bumped by every flk_set_curves
   // scratch
   flk::DevBuf d_pairs, d_tasks, d_cost, d_task_theta, d_task_corr, d_pair_task0,
       d_pair_out, d_pair_corr, d_boxes, d_pls, d_count, d_misc;
@@ -186,7 +213,14 @@ namespace flk {
 // per-segment-pair half solid angles (radians) and corr_out[k] the integer
 // branch corrections; the linking number is theta/(2 pi) + corr.
 void ds_run(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, bool fp64,
-            std::vector<double> &lk_out, float *ms_kernel);
+            std::vector<double> &lk_out, float *ms_kernel, bool use_graph = false);
+// Direct-sum kernel time (ms) for every `stride`-th 64R-row block of the
+// pairs' work with launch shape `t` (FP32); *work receives the segment pairs
+// of the sample.  Results are discarded.
+float ds_time_sample(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, const flk_tune &t, int64_t stride,
+                     int64_t *work);
+// Segment pairs of the pairs' direct-sum work.
+int64_t ds_work(flk_ctx *ctx, const int32_t *pairs, int64_t npairs);
 
 void bh_run(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, const flk_params &prm,
             std::vector<double> &lk, std::vector<double> &err, std::vector<uint32_t> &flags);

@@ -204,6 +204,14 @@ void LinkingNumberCertificate::ComputeFromModel(const Model &model, bool force_d
   }
   std::vector<double> lk(pairs.size()), err(pairs.size());
   std::vector<uint32_t> flags(pairs.size());
+  if (opt.autotune) {
+    flk_tune t;
+    engine_check(ctx, flk_autotune(ctx, flat.data(), (int64_t)pairs.size(), &prm, &t), "flk_autotune");
+    if (v)
+      std::cout << "Autotuned shapes: direct sum " << t.ds_rows_per_lane << " rows per lane, "
+                << t.ds_blocks_per_cu << " blocks per CU (0 = all resident), " << t.ds_tasks_per_wave
+                << " tasks per wave; Barnes–Hut " << t.bh_items_per_thread << " items per thread." << std::endl;
+  }
   engine_check(ctx, flk_linking_numbers(ctx, flat.data(), (int64_t)pairs.size(), &prm, lk.data(),
                                         err.data(), flags.data()),
                "flk_linking_numbers");

@@ -34,6 +34,7 @@ int main(int argc, char **argv) {
   F.add("device", "0", "GPU device index");
   F.add("mlp_log_tol", "-6", "learned criterion: accept far field below this predicted log10 error");
   F.add("mlp_weights", "", "learned criterion weight file (default: the package's mlp_criterion.bin)");
+  F.add("autotune", "false", "time candidate kernel launch shapes on this model's pairs first", true);
   try {
     F.parse(argc, argv);
   } catch (const std::exception &e) {
@@ -63,6 +64,7 @@ int main(int argc, char **argv) {
   opt.device = (int)F.num("device");
   opt.mlp_log_tol = F.num("mlp_log_tol");
   opt.mlp_weights = F.str("mlp_weights");
+  opt.autotune = F.flag("autotune");
   LinkingNumberCertificate certificate;
   {
     std::cout << "Computing the certificate using the " << (ds ? "Direct Summation" : "Barnes–Hut")

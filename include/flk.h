@@ -63,7 +63,10 @@ typedef struct flk_params {
   int32_t timing;       /* 1: time every Barnes–Hut level and prediction launch
                            with HIP events (ms_bh_level, ms_mlp); the per-launch
                            events cost ~15% of a traversal, so 0 by default */
-  int32_t reserved[6];
+  int32_t graph;        /* 1 (default): a direct-sum batch is captured in a HIP
+                           graph and replayed when the same pairs come again;
+                           0: launched eagerly every time */
+  int32_t reserved[5];
 } flk_params;
 
 typedef struct flk_stats {
@@ -132,6 +135,32 @@ int32_t flk_linking_numbers(flk_ctx *ctx, const int32_t *pairs, int64_t npairs,
 
 /* Statistics of the last flk_linking_numbers call. */
 int32_t flk_get_stats(const flk_ctx *ctx, flk_stats *out);
+
+/* Launch shapes of the kernels.  No reference counterpart (the reference is
+ * CPU-only): flk_autotune picks them by timing candidates on the caller's own
+ * pairs, flk_set_tune installs a saved choice.  Every shape gives the same
+ * results up to floating-point summation order. */
+typedef struct flk_tune {
+  int32_t ds_rows_per_lane;    /* FP32 direct-sum tile: rows per lane, 4 or 8
+                                  (a wave owns 64 * R consecutive rows) */
+  int32_t ds_blocks_per_cu;    /* persistent direct-sum grid: 256-thread blocks
+                                  per CU (0: as many as are resident at once) */
+  int32_t ds_tasks_per_wave;   /* column chunking: target tasks per wave */
+  int32_t bh_items_per_thread; /* Barnes–Hut level and leaf kernels: 2, 4 or 8 */
+  int32_t reserved[4];
+} flk_tune;
+
+int32_t flk_get_tune(const flk_ctx *ctx, flk_tune *out);
+int32_t flk_set_tune(flk_ctx *ctx, const flk_tune *tune);
+
+/* Times candidate shapes for these pairs and parameters -- the direct-sum
+ * shapes on a deterministic sample of the pairs' direct-sum work (every k-th
+ * 64R-row block, about a tenth of it), the Barnes–Hut shapes on the
+ * Barnes–Hut pairs -- keeps the
+ * fastest in the context and writes it to *out (NULL allowed).  The
+ * linking numbers themselves are not returned. */
+int32_t flk_autotune(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, const flk_params *params,
+                     flk_tune *out);
 
 /* Learned opening criterion (no reference counterpart; it replaces the beta
  * test of src/compute_link.cpp:573-574 and :608).  Weights are FP32 masters of

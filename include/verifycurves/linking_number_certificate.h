@@ -29,6 +29,7 @@ struct EngineOptions {
   bool verbose = true;             // print the reference's progress lines
   double mlp_log_tol = -6.0;       // learned criterion: far field below this log10 error
   std::string mlp_weights;         // weight file; empty = mlp_criterion.bin of the package
+  bool autotune = false;           // time candidate kernel shapes on this model's pairs first
 };
 
 class LinkingNumberCertificate {

@@ -88,3 +88,11 @@ def test_learned_criterion_cli(tmp_path):
                    "--criterion=mlp"])
     assert out.read_text() == "39\n5,6,-2\n"
     assert "Mean Off-Integer Error" in stdout
+
+
+def test_autotune_cli(tmp_path):
+    out = tmp_path / "kb.txt"
+    stdout = _run([f"--input={os.path.join(DATA, 'knittubebroken.bcc')}", f"--output={out}",
+                   "--method=directsum", "--autotune"])
+    assert out.read_text() == "39\n5,6,-2\n"
+    assert "Autotuned shapes: direct sum" in stdout

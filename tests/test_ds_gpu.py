@@ -100,3 +100,74 @@ def test_degenerate_curves(engine):
     assert round(res.lk[0]) == -1
     assert res.lk[1] == 0.0 and res.lk[2] == 0.0
     assert np.isfinite(res.lk[3])
+
+
+def test_tile_shapes_and_autotune(engine):
+    """Every direct-sum and traversal launch shape gives the same certificate
+    (sums agree to FP32 summation order); the autotuner times the candidates
+    on the caller's pairs, keeps a valid shape and leaves the statistics of
+    the last computation alone."""
+    default = engine.get_tune()
+    try:
+        _check_shapes(engine)
+    finally:
+        engine.set_tune(**default)
+
+
+def _check_shapes(engine):
+    curves = W.workload(2, scale=0.05)
+    engine.set_curves(curves)
+    pairs = engine.potential_links()
+    base = engine.linking_numbers(pairs, method="directsum", fallback_tol=0.0)
+    cert = flk.certificate_from_lk(pairs, base.lk)
+    for R in (4, 8):
+        for bpc in (0, 2, 4):
+            engine.set_tune(ds_rows_per_lane=R, ds_blocks_per_cu=bpc, ds_tasks_per_wave=4)
+            r = engine.linking_numbers(pairs, method="directsum", fallback_tol=0.0)
+            assert flk.certificate_from_lk(pairs, r.lk) == cert
+            np.testing.assert_allclose(r.lk, base.lk, rtol=0, atol=2e-5)
+    bh0 = engine.linking_numbers(pairs, method="barneshut", fallback_tol=0.0)
+    for ipt in (2, 8):
+        engine.set_tune(bh_items_per_thread=ipt)
+        r = engine.linking_numbers(pairs, method="barneshut", fallback_tol=0.0)
+        assert r.stats["bh_visits"] == bh0.stats["bh_visits"]
+        np.testing.assert_allclose(r.lk, bh0.lk, rtol=0, atol=1e-9)
+    with pytest.raises(RuntimeError):
+        engine.set_tune(ds_rows_per_lane=5)
+    before = engine.stats()
+    t = engine.autotune(pairs, method="barneshut", bh_direct_threshold=300 * 300)
+    print("tuned", t)
+    assert t["ds_rows_per_lane"] in (4, 8) and t["bh_items_per_thread"] in (2, 4, 8)
+    assert t["ds_blocks_per_cu"] in (0, 4) and t["ds_tasks_per_wave"] in (8, 16)
+    assert engine.get_tune() == t
+    assert engine.stats() == before
+    r = engine.linking_numbers(pairs, method="directsum", fallback_tol=0.0)
+    assert flk.certificate_from_lk(pairs, r.lk) == cert
+
+
+def test_graph_replays_match_eager(engine):
+    """The batched direct sum is captured in a HIP graph and replayed for a
+    repeated pair set.  Each replay, with other eager work and
+    synchronisations in between, must reproduce the eagerly launched batch
+    bit for bit; a new curve set must not replay a stale graph."""
+    curves = W.workload(2, scale=0.05)
+    engine.set_curves(curves)
+    pairs = engine.potential_links()
+    eager = engine.linking_numbers(pairs, method="directsum", fallback_tol=0.0, graph=0).lk
+    other = pairs[: max(1, len(pairs) // 3)]
+    other_eager = engine.linking_numbers(other, method="directsum", fallback_tol=0.0, graph=0).lk
+    for rep in range(4):
+        r = engine.linking_numbers(pairs, method="directsum", fallback_tol=0.0)  # capture, then replays
+        assert np.array_equal(r.lk, eager), rep
+        assert r.stats["ds_seg_pairs"] > 0
+        # eager work of the program between replays
+        o = engine.linking_numbers(other, method="directsum", fallback_tol=0.0, graph=0)
+        assert np.array_equal(o.lk, other_eager)
+        engine.linking_numbers(pairs[:2], method="barneshut", fallback_tol=0.0)
+    # new curves (shifted copies): same pair indices, different sums expected
+    moved = [c + np.float32(0.25) * (i % 2) for i, c in enumerate(curves)]
+    engine.set_curves(moved)
+    fresh = engine.linking_numbers(pairs, method="directsum", fallback_tol=0.0, graph=0).lk
+    again = engine.linking_numbers(pairs, method="directsum", fallback_tol=0.0).lk
+    assert np.array_equal(again, fresh)
+    assert not np.array_equal(fresh, eager)
