@@ -414,16 +414,17 @@ void ds_plan(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, bool fp64, cons
   Curves &cv = ctx->curves;
   P.fp64 = fp64;
   P.precise = cv.inexact;
-  P.R = fp64 ? kRowsPerLane64 : (t.ds_rows_per_lane == 4 ? 4 : 8);
+  P.R = fp64 ? kRowsPerLane64 : (t.ds_rows_per_lane == 2 || t.ds_rows_per_lane == 4 ? t.ds_rows_per_lane : 8);
   const int rows_per_wave = 64 * P.R;
   int bpc = t.ds_blocks_per_cu;
   if (bpc <= 0) {
-    static int cached[5] = {0, 0, 0, 0, 0};  // f64, f32 R4 exact/precise, R8 exact/precise
-    const int slot = fp64 ? 0 : 1 + (P.R == 8 ? 2 : 0) + (P.precise ? 1 : 0);
+    static int cached[7] = {0, 0, 0, 0, 0, 0, 0};  // f64; f32 R2, R4, R8 x exact/precise
+    const int slot = fp64 ? 0 : 1 + 2 * (P.R == 2 ? 0 : P.R == 4 ? 1 : 2) + (P.precise ? 1 : 0);
     if (!cached[slot]) {
-      const void *k = fp64 ? (const void *)ds_tile_f64<kRowsPerLane64>
-                    : P.R == 4 ? (P.precise ? (const void *)ds_tile_f32<4, true> : (const void *)ds_tile_f32<4, false>)
-                               : (P.precise ? (const void *)ds_tile_f32<8, true> : (const void *)ds_tile_f32<8, false>);
+      const void *k = fp64        ? (const void *)ds_tile_f64<kRowsPerLane64>
+                      : P.R == 2 ? (P.precise ? (const void *)ds_tile_f32<2, true> : (const void *)ds_tile_f32<2, false>)
+                      : P.R == 4 ? (P.precise ? (const void *)ds_tile_f32<4, true> : (const void *)ds_tile_f32<4, false>)
+                                 : (P.precise ? (const void *)ds_tile_f32<8, true> : (const void *)ds_tile_f32<8, false>);
       cached[slot] = resident_blocks(k);
     }
     bpc = cached[slot];
@@ -514,6 +515,13 @@ void ds_enqueue(flk_ctx *ctx, const DsPlan &P, const DsPair *dp, const DsTask *t
   if (P.fp64) {
     hipLaunchKernelGGL((ds_tile_f64<kRowsPerLane64>), grid, block, 0, s, cv.dpts.as<double4>(), d.pairs, d.tasks,
                        d.cost, ntasks, d.theta, d.corr);
+  } else if (P.R == 2) {
+    if (P.precise)
+      hipLaunchKernelGGL((ds_tile_f32<2, true>), grid, block, 0, s, hi, lo, ed, d.pairs, d.tasks, d.cost, ntasks,
+                         d.theta, d.corr);
+    else
+      hipLaunchKernelGGL((ds_tile_f32<2, false>), grid, block, 0, s, hi, lo, ed, d.pairs, d.tasks, d.cost, ntasks,
+                         d.theta, d.corr);
   } else if (P.R == 4) {
     if (P.precise)
       hipLaunchKernelGGL((ds_tile_f32<4, true>), grid, block, 0, s, hi, lo, ed, d.pairs, d.tasks, d.cost, ntasks,

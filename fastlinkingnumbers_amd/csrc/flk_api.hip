@@ -301,8 +301,8 @@ int32_t flk_get_tune(const flk_ctx *ctx, flk_tune *out) {
 int32_t flk_set_tune(flk_ctx *ctx, const flk_tune *tune) {
   if (!ctx || !tune) return FLK_ERR_ARG;
   FLK_GUARD(ctx, {
-    if (tune->ds_rows_per_lane != 4 && tune->ds_rows_per_lane != 8)
-      throw Error(FLK_ERR_ARG, "ds_rows_per_lane must be 4 or 8");
+    if (tune->ds_rows_per_lane != 2 && tune->ds_rows_per_lane != 4 && tune->ds_rows_per_lane != 8)
+      throw Error(FLK_ERR_ARG, "ds_rows_per_lane must be 2, 4 or 8");
     if (tune->ds_blocks_per_cu < 0 || tune->ds_blocks_per_cu > 64)
       throw Error(FLK_ERR_ARG, "ds_blocks_per_cu must be in [0, 64]");
     if (tune->ds_tasks_per_wave < 1 || tune->ds_tasks_per_wave > 1024)
@@ -347,7 +347,7 @@ int32_t flk_autotune(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, const f
       double best_rate = -1.0;
       // every stride-th row block: about target / total of the work
       const int64_t stride = std::max<int64_t>(1, total / target);
-      for (int R : {4, 8}) {
+      for (int R : {2, 4, 8}) {
         for (int bpc : {0, 4}) {  // all blocks resident, or 4 per CU
           for (int tpw : {8, 16}) {
             flk_tune t = ctx->tune;

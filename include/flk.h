@@ -141,7 +141,7 @@ int32_t flk_get_stats(const flk_ctx *ctx, flk_stats *out);
  * pairs, flk_set_tune installs a saved choice.  Every shape gives the same
  * results up to floating-point summation order. */
 typedef struct flk_tune {
-  int32_t ds_rows_per_lane;    /* FP32 direct-sum tile: rows per lane, 4 or 8
+  int32_t ds_rows_per_lane;    /* FP32 direct-sum tile: rows per lane, 2, 4 or 8
                                   (a wave owns 64 * R consecutive rows) */
   int32_t ds_blocks_per_cu;    /* persistent direct-sum grid: 256-thread blocks
                                   per CU (0: as many as are resident at once) */

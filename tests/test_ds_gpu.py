@@ -120,7 +120,7 @@ def _check_shapes(engine):
     pairs = engine.potential_links()
     base = engine.linking_numbers(pairs, method="directsum", fallback_tol=0.0)
     cert = flk.certificate_from_lk(pairs, base.lk)
-    for R in (4, 8):
+    for R in (2, 4, 8):
         for bpc in (0, 2, 4):
             engine.set_tune(ds_rows_per_lane=R, ds_blocks_per_cu=bpc, ds_tasks_per_wave=4)
             r = engine.linking_numbers(pairs, method="directsum", fallback_tol=0.0)
@@ -137,7 +137,7 @@ def _check_shapes(engine):
     before = engine.stats()
     t = engine.autotune(pairs, method="barneshut", bh_direct_threshold=300 * 300)
     print("tuned", t)
-    assert t["ds_rows_per_lane"] in (4, 8) and t["bh_items_per_thread"] in (2, 4, 8)
+    assert t["ds_rows_per_lane"] in (2, 4, 8) and t["bh_items_per_thread"] in (2, 4, 8)
     assert t["ds_blocks_per_cu"] in (0, 4) and t["ds_tasks_per_wave"] in (8, 16)
     assert engine.get_tune() == t
     assert engine.stats() == before
