@@ -96,6 +96,24 @@ class FlkTune(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
+class FlkDiscInfo(C.Structure):
+    """Counts of a device discretization (include/flk.h flk_disc_info)."""
+    _fields_ = [
+        ("segments", C.c_int64),
+        ("vertices", C.c_int64),
+        ("rounds", C.c_int32),
+        ("small_curve", C.c_int32),
+        ("small_segment", C.c_int32),
+        ("pad", C.c_int32),
+        ("small_size", C.c_double),
+        ("overlaps", C.c_int64 * 55),
+        ("pieces", C.c_int64),
+        ("reserved", C.c_int64 * 3),
+    ]
+
+
+SEG_BSPLINE, SEG_CATMULLROM, SEG_BEZIER, SEG_POLYLINE = 0, 1, 2, 3
+
 FLK_EXPORTS = [
     ("flk_default_params", C.c_int32, [_P(FlkParams)]),
     ("flk_create", C.c_int32, [C.c_int32, _P(C.c_void_p)]),
@@ -110,6 +128,9 @@ FLK_EXPORTS = [
      [C.c_void_p, _P(C.c_int32), C.c_int64, _P(FlkParams), _P(C.c_double), _P(C.c_double),
       _P(C.c_uint32)]),
     ("flk_get_stats", C.c_int32, [C.c_void_p, _P(FlkStats)]),
+    ("flk_discretize", C.c_int32,
+     [C.c_void_p, _P(C.c_double), _P(C.c_int64), _P(C.c_int32), _P(C.c_uint8), C.c_int32, _P(C.c_int32),
+      C.c_int64, _P(_P(C.c_double)), _P(_P(C.c_int64)), _P(FlkDiscInfo)]),
     ("flk_get_tune", C.c_int32, [C.c_void_p, _P(FlkTune)]),
     ("flk_set_tune", C.c_int32, [C.c_void_p, _P(FlkTune)]),
     ("flk_autotune", C.c_int32, [C.c_void_p, _P(C.c_int32), C.c_int64, _P(FlkParams), _P(FlkTune)]),
@@ -277,6 +298,31 @@ class Engine:
         s = FlkStats()
         self._check(self.lib.flk_get_stats(self.h, C.byref(s)), "flk_get_stats")
         return s.as_dict()
+
+    # ---- discretization
+    def discretize(self, curves, types, looped, pairs):
+        """Device discretization of control polygons `curves` (segment types
+        SEG_*, looped flags) against potentially linked `pairs`; returns the
+        polylines and the counts (rounds, overlaps per round, pieces)."""
+        n = len(curves)
+        types = np.broadcast_to(np.asarray(types, dtype=np.int32), (n,)).copy()
+        looped = np.broadcast_to(np.asarray(looped, dtype=np.uint8), (n,)).copy()
+        xyz, offs = pack_curves(curves)
+        pairs = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1, 2))
+        out = C.POINTER(C.c_double)()
+        ooff = C.POINTER(C.c_int64)()
+        info = FlkDiscInfo()
+        rc = self.lib.flk_discretize(self.h, _ptr(xyz, C.c_double), _ptr(offs, C.c_int64), _ptr(types, C.c_int32),
+                                     _ptr(looped, C.c_uint8), n, _ptr(pairs, C.c_int32), len(pairs),
+                                     C.byref(out), C.byref(ooff), C.byref(info))
+        d = {"segments": info.segments, "vertices": info.vertices, "rounds": info.rounds,
+             "pieces": info.pieces, "overlaps": list(info.overlaps[: info.rounds]),
+             "small_curve": info.small_curve, "small_segment": info.small_segment,
+             "small_size": info.small_size}
+        self._check(rc, "flk_discretize")
+        off = np.ctypeslib.as_array(ooff, shape=(n + 1,)).copy()
+        v = np.ctypeslib.as_array(out, shape=(3 * int(off[-1]),)).copy().reshape(-1, 3) if off[-1] else np.zeros((0, 3))
+        return [v[off[c]:off[c + 1]] for c in range(n)], d
 
     # ---- launch shapes
     def get_tune(self) -> dict:

@@ -383,7 +383,6 @@ void convert_vertices_launch(hipStream_t s, const double4 *d, int64_t n, float4 
 }
 
 // Rows per lane of the FP32 tile kernel, and rows per wave.
-constexpr int kRowsPerLane = 8;
 constexpr int kRowsPerLane64 = 2;
 
 namespace {

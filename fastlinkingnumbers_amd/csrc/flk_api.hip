@@ -292,6 +292,49 @@ int32_t flk_get_stats(const flk_ctx *ctx, flk_stats *out) {
   return FLK_OK;
 }
 
+int32_t flk_discretize(flk_ctx *ctx, const double *ctrl_xyz, const int64_t *offsets,
+                       const int32_t *segment_types, const uint8_t *looped, int32_t ncurves,
+                       const int32_t *pairs, int64_t npairs, const double **out_xyz,
+                       const int64_t **out_offsets, flk_disc_info *info) {
+  if (!ctx || ncurves < 0 || npairs < 0 || !out_xyz || !out_offsets) return FLK_ERR_ARG;
+  if (ncurves > 0 && (!offsets || !segment_types || !looped)) return FLK_ERR_ARG;
+  if (npairs > 0 && !pairs) return FLK_ERR_ARG;
+  auto fill = [&]() {
+    if (!info) return;
+    std::memset(info, 0, sizeof *info);
+    const flk::DiscResult &r = ctx->disc;
+    info->segments = r.segments;
+    info->vertices = r.out_offsets.empty() ? 0 : r.out_offsets.back();
+    info->rounds = r.rounds;
+    info->pieces = r.retired;
+    info->small_curve = r.small_curve;
+    info->small_segment = r.small_segment;
+    info->small_size = r.small_size;
+    for (size_t k = 0; k < r.overlaps.size() && k < 55; ++k) info->overlaps[k] = r.overlaps[k];
+  };
+  try {
+    FLK_HIP(hipSetDevice(ctx->device));
+    if (ncurves > 0 && offsets[0] != 0) throw Error(FLK_ERR_ARG, "offsets[0] must be 0");
+    if (ncurves > 0 && offsets[ncurves] > 0 && !ctrl_xyz) throw Error(FLK_ERR_ARG, "ctrl_xyz is NULL");
+    flk::discretize_run(ctx, ctrl_xyz, offsets, segment_types, looped, ncurves, pairs, npairs, ctx->disc);
+  } catch (const flk::Error &e) {
+    ctx->disc.rounds = (int32_t)ctx->disc.overlaps.size();
+    fill();
+    ctx->err = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    ctx->err = "host allocation failed";
+    return FLK_ERR_NOMEM;
+  } catch (const std::exception &e) {
+    ctx->err = e.what();
+    return FLK_ERR_STATE;
+  }
+  fill();
+  *out_xyz = ctx->disc.xyz.data();
+  *out_offsets = ctx->disc.out_offsets.data();
+  return FLK_OK;
+}
+
 int32_t flk_get_tune(const flk_ctx *ctx, flk_tune *out) {
   if (!ctx || !out) return FLK_ERR_ARG;
   *out = ctx->tune;

@@ -176,6 +176,17 @@ struct DsGraph {
   ~DsGraph() { reset(); }
 };
 
+// Result of a device discretization (context-owned).
+struct DiscResult {
+  int64_t segments = 0, retired = 0;
+  int32_t rounds = 0;
+  std::vector<int64_t> overlaps;     // overlapping piece pairs per round
+  std::vector<int64_t> out_offsets;  // ncurves + 1, into xyz / 3
+  std::vector<double> xyz;           // polyline vertices, AoS
+  int32_t small_curve = -1, small_segment = -1;  // the input guard's smallest segment
+  double small_size = 0.0;
+};
+
 struct MlpState {
   std::vector<float> params;  // FLK_MLP_NPARAM FP32 masters
   std::vector<float> norm;    // mean[12], std[12]
@@ -201,6 +212,7 @@ struct flk_ctx {
   // every block resident: 985 ms against 1,016 ms for R = 8)
   flk_tune tune{4, 0, 8, 4, {0, 0, 0, 0}};
   flk::DsGraph ds_graph[2];      // FP32, FP64 batches
+  flk::DiscResult disc;          // last flk_discretize result
   uint64_t curves_version = 0;   // bumped by every flk_set_curves
   // scratch
   flk::DevBuf d_pairs, d_tasks, d_cost, d_task_theta, d_task_corr, d_pair_task0,
@@ -221,6 +233,11 @@ float ds_time_sample(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, const f
                      int64_t *work);
 // Segment pairs of the pairs' direct-sum work.
 int64_t ds_work(flk_ctx *ctx, const int32_t *pairs, int64_t npairs);
+
+// Spline-to-polyline discretization of the given curves against the given
+// potentially linked pairs (see disc.hip).
+void discretize_run(flk_ctx *ctx, const double *ctrl, const int64_t *offsets, const int32_t *types,
+                    const uint8_t *looped, int32_t nc, const int32_t *pairs, int64_t npairs, DiscResult &res);
 
 void bh_run(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, const flk_params &prm,
             std::vector<double> &lk, std::vector<double> &err, std::vector<uint32_t> &flags);

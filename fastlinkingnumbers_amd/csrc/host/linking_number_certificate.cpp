@@ -118,6 +118,74 @@ std::vector<std::pair<int, int>> potential_links(flk_ctx *ctx, const std::vector
   return out;
 }
 
+// Discretize (reference src/discretize.cpp:102-281) on the device: the
+// engine bisects the pieces and evaluates the polyline vertices; this prints
+// the reference's progress and failure messages and builds the curves.
+std::vector<Curve> discretize_curves(flk_ctx *ctx, const std::vector<Curve> &in,
+                                     const std::vector<std::pair<int, int>> &links, bool verbose) {
+  const int32_t nc = (int32_t)in.size();
+  std::vector<int64_t> off(nc + 1, 0);
+  std::vector<int32_t> types(nc);
+  std::vector<uint8_t> looped(nc);
+  for (int32_t c = 0; c < nc; ++c) {
+    off[c + 1] = off[c] + (int64_t)in[c].get_q().size();
+    types[c] = static_cast<int32_t>(in[c].get_segment_type());
+    looped[c] = in[c].is_looped() ? 1 : 0;
+  }
+  std::vector<double> ctrl(3 * (size_t)off[nc]);
+  for (int32_t c = 0; c < nc; ++c)
+    for (size_t k = 0; k < in[c].get_q().size(); ++k)
+      for (int d = 0; d < 3; ++d) ctrl[3 * (off[c] + (int64_t)k) + d] = in[c].get_q()[k][d];
+  std::vector<int32_t> pairs(2 * links.size());
+  for (size_t k = 0; k < links.size(); ++k) {
+    pairs[2 * k] = links[k].first;
+    pairs[2 * k + 1] = links[k].second;
+  }
+  const double *xyz = nullptr;
+  const int64_t *voff = nullptr;
+  flk_disc_info info;
+  const int32_t rc = flk_discretize(ctx, ctrl.data(), off.data(), types.data(), looped.data(), nc, pairs.data(),
+                                    (int64_t)links.size(), &xyz, &voff, &info);
+  if (verbose)
+    for (int32_t r = 0; r < info.rounds && r < 55; ++r)
+      if (rc == FLK_OK || r + 1 < info.rounds || info.small_curve >= 0 || info.rounds == 55)
+        std::cout << "Iteration " << r + 1 << ", " << info.overlaps[r] << " overlaps." << std::endl;
+  if (rc != FLK_OK) {
+    const std::string msg = flk_last_error(ctx);
+    if (info.small_curve >= 0) {
+      std::cerr << "Discretize: Smallest input segment, (" << info.small_curve << "," << info.small_segment
+                << ") has a size of " << info.small_size
+                << ", which is smaller than eps * avg coordinate. Consider combining them into one vertex."
+                << std::endl;
+    } else if (msg.find("Smallest segment") != std::string::npos) {
+      std::cout << "Discretize: Smallest segment is smaller than eps * avg coordinate. Perhaps curves are too close."
+                << std::endl;
+    } else if (msg.find("max recursion") != std::string::npos) {
+      std::cout << "Max recursion (curves are too close)." << std::endl;
+      std::cout << "Number of overlaps: " << info.overlaps[54] << std::endl;
+    }
+    throw std::runtime_error(msg);
+  }
+  if (verbose) {
+    std::cout << "Finished after " << info.rounds << " iteration(s)." << std::endl;
+    std::cout << "Original segment count: " << info.segments << ", discretized count: " << info.pieces
+              << std::endl;
+  }
+  std::vector<Curve> out(nc);
+  for (int32_t c = 0; c < nc; ++c) {
+    Curve &o = out[c];
+    o.set_segment_type(SegmentType::Polyline);
+    const int64_t n = voff[c + 1] - voff[c];
+    o.get_q().resize((size_t)n);
+    for (int64_t k = 0; k < n; ++k)
+      o.get_q()[(size_t)k] = Point3(xyz[3 * (voff[c] + k)], xyz[3 * (voff[c] + k) + 1], xyz[3 * (voff[c] + k) + 2]);
+    // an open input curve with pieces stays open (its last end point closes the list)
+    o.set_looped(in[c].is_looped() || n == 0);
+    o.ComputeCurveBoundingBox();
+  }
+  return out;
+}
+
 void save_entries(const std::string &filename, int size,
                   const std::vector<LinkingNumberCertificate::Entry> &e, bool omit_size) {
   std::ofstream f(filename);
@@ -165,14 +233,8 @@ void LinkingNumberCertificate::ComputeFromModel(const Model &model, bool force_d
   for (const auto &c : in) all_pl = all_pl && c.get_segment_type() == SegmentType::Polyline;
   if (discretize) {
     auto first = potential_links(ctx, in, false);
-    std::vector<std::vector<int>> links(nc);
-    for (auto &p : first) {
-      links[p.first].push_back(p.second);
-      links[p.second].push_back(p.first);
-    }
-    for (auto &l : links) std::sort(l.begin(), l.end());
     if (v) std::cout << "Starting Discretization." << std::endl;
-    disc = Discretize(in, links, v);
+    disc = discretize_curves(ctx, in, first, v);
     curves = &disc;
     upload(ctx, disc);
     pairs = potential_links(ctx, disc, true);

@@ -136,6 +136,42 @@ int32_t flk_linking_numbers(flk_ctx *ctx, const int32_t *pairs, int64_t npairs,
 /* Statistics of the last flk_linking_numbers call. */
 int32_t flk_get_stats(const flk_ctx *ctx, flk_stats *out);
 
+/* Segment types of flk_discretize, in the order of the reference's
+ * SegmentType (include/curve.h). */
+#define FLK_SEG_BSPLINE 0
+#define FLK_SEG_CATMULLROM 1
+#define FLK_SEG_BEZIER 2
+#define FLK_SEG_POLYLINE 3
+
+typedef struct flk_disc_info {
+  int64_t segments;         /* input segments */
+  int64_t vertices;         /* output vertices over all curves */
+  int32_t rounds;           /* bisection rounds run */
+  int32_t small_curve;      /* when the input guard fails: the smallest segment */
+  int32_t small_segment;
+  int32_t pad;
+  double small_size;        /* and its box diagonal */
+  int64_t overlaps[55];     /* overlapping piece pairs found in each round */
+  int64_t pieces;           /* output polyline segments over all curves */
+  int64_t reserved[3];
+} flk_disc_info;
+
+/* Replaces Discretize (src/discretize.cpp:102-281): bisects the pieces of
+ * every segment whose exact box overlaps a piece of a potentially linked
+ * curve (pairs: i < j, as from flk_potential_links on the curves' spline
+ * boxes) until no overlap is left, then returns each curve as a polyline of
+ * its pieces' start points (an open curve also gets its last end point).
+ * ctrl_xyz: control points, AoS, curve c = rows offsets[c]..offsets[c+1];
+ * segment_types: FLK_SEG_*; looped: 0/1 per curve.  *out_xyz / *out_offsets
+ * are context-owned (valid until the next call); *info (may be NULL) gets the
+ * counts and, on the reference's failures (smallest input segment below
+ * eps * mean |q| / sqrt(3), a new piece below it, 55 rounds with overlaps
+ * left), the error details (status FLK_ERR_ARG, message as the reference's). */
+int32_t flk_discretize(flk_ctx *ctx, const double *ctrl_xyz, const int64_t *offsets,
+                       const int32_t *segment_types, const uint8_t *looped, int32_t ncurves,
+                       const int32_t *pairs, int64_t npairs, const double **out_xyz,
+                       const int64_t **out_offsets, flk_disc_info *info);
+
 /* Launch shapes of the kernels.  No reference counterpart (the reference is
  * CPU-only): flk_autotune picks them by timing candidates on the caller's own
  * pairs, flk_set_tune installs a saved choice.  Every shape gives the same
