@@ -210,12 +210,16 @@ __device__ void store_node(const Mom64 &M, int2 ch, BhRange r, BhGeo *__restrict
 }
 
 // Bottom-up moments of one level (children already done) and the node's FP32
-// records.  Leaf buckets keep no FP64 record: a parent adds a leaf child's
-// segments about its own centre directly (the closed form shifted exactly),
-// an internal child's record with the shift formulas of the reference
-// (compute_link.cpp:358-389, Eqs. 21-23).  Children are read one at a time.
+// records.  A node of at most `direct` segments (leaf buckets and the small
+// subtrees above them, which are most of the nodes) sums its segments' closed
+// forms about its own centre and keeps no FP64 record; a larger node adds
+// such a child's segments about its own centre directly (the closed form
+// shifted exactly) and a larger child's FP64 record with the shift formulas
+// of the reference (compute_link.cpp:358-389, Eqs. 21-23).  Only nodes above
+// `direct` segments write FP64 records, so the bottom levels never round-trip
+// them through memory.  Children are read one at a time.
 __global__ __launch_bounds__(128) void bh_moments(const BhRange *__restrict__ range, int32_t n,
-                                                  int32_t base, int32_t leaf,
+                                                  int32_t base, int32_t direct,
                                                   const int2 *__restrict__ child,
                                                   const uint32_t *__restrict__ order,
                                                   const double4 *__restrict__ pts,
@@ -230,7 +234,7 @@ __global__ __launch_bounds__(128) void bh_moments(const BhRange *__restrict__ ra
   for (int t = 0; t < 3; ++t) M.c[t] = 0;
   for (int t = 0; t < 9; ++t) M.D[t] = 0;
   for (int t = 0; t < 18; ++t) M.Q[t] = 0;
-  if (ch.x < 0) {
+  if (ch.x < 0 || rr.hi - rr.lo <= direct) {
     leaf_box(rr, order, pts, M.box);
     double ctr[3];
     for (int d = 0; d < 3; ++d) ctr[d] = 0.5 * (M.box[3 + d] + M.box[d]);
@@ -240,7 +244,7 @@ __global__ __launch_bounds__(128) void bh_moments(const BhRange *__restrict__ ra
     const int cid[2] = {ch.x, ch.y};
     double cb[2][6];
     for (int side = 0; side < 2; ++side) {
-      if (rc[side].hi - rc[side].lo <= leaf) leaf_box(rc[side], order, pts, cb[side]);
+      if (rc[side].hi - rc[side].lo <= direct) leaf_box(rc[side], order, pts, cb[side]);
       else
         for (int t = 0; t < 6; ++t) cb[side][t] = m64[cid[side]].box[t];
     }
@@ -251,7 +255,7 @@ __global__ __launch_bounds__(128) void bh_moments(const BhRange *__restrict__ ra
     double ctr[3];
     for (int d = 0; d < 3; ++d) ctr[d] = 0.5 * (M.box[3 + d] + M.box[d]);
     for (int side = 0; side < 2; ++side) {
-      if (rc[side].hi - rc[side].lo <= leaf) {
+      if (rc[side].hi - rc[side].lo <= direct) {
         add_segments(rc[side], order, pts, ctr, M);
         continue;
       }
@@ -771,6 +775,10 @@ __global__ void bh_roots(const int32_t *__restrict__ ra, const int32_t *__restri
 namespace {
 
 // Node records of the trees of one call; the buffers are the context's.
+// Nodes of at most this many segments take their moments from their segments
+// (no FP64 record round trip; a node reads at most 2 x 8 segments).
+constexpr int32_t kDirectSegments = 8;
+
 struct Forest {
   int32_t nnodes = 0;
   std::vector<int32_t> root;  // per curve (-1 if no tree)
@@ -839,23 +847,57 @@ void build_forest(flk_ctx *ctx, const std::vector<int32_t> &curves_in, int leaf,
   d_flag.reserve(sizeof(int32_t) * (S + nc + 1));
   d_scan.reserve(sizeof(int32_t) * (S + nc + 1));
   FLK_HIP(hipMemcpyAsync(d_range.ptr, roots.data(), sizeof(BhRange) * nc, hipMemcpyHostToDevice, s));
+  // Splits per level from the segment counts alone: a range of m > leaf
+  // segments splits into floor(m/2) and m - floor(m/2), so every level of a
+  // curve's tree holds at most two distinct range sizes (m and m + 1).  With
+  // the counts known, the levels are enqueued without reading anything back.
+  std::vector<int64_t> level_split;
+  {
+    std::vector<std::pair<int64_t, int64_t>> sizes;  // (range size, count) of the current level
+    std::vector<std::pair<int64_t, int64_t>> next;
+    for (int32_t k = 0; k < nc; ++k) sizes.emplace_back(seg_off[k + 1] - seg_off[k], 1);
+    for (;;) {
+      int64_t split = 0;
+      next.clear();
+      for (const auto &sc : sizes) {
+        if (sc.first <= leaf) continue;
+        split += sc.second;
+        next.emplace_back(sc.first / 2, sc.second);
+        next.emplace_back(sc.first - sc.first / 2, sc.second);
+      }
+      level_split.push_back(split);
+      if (split == 0) break;
+      // merge equal sizes (keeps the list at two entries per distinct size pair)
+      std::sort(next.begin(), next.end());
+      sizes.clear();
+      for (const auto &sc : next) {
+        if (!sizes.empty() && sizes.back().first == sc.first) sizes.back().second += sc.second;
+        else sizes.push_back(sc);
+      }
+    }
+  }
+  // scan scratch for the widest level, reserved once: nothing is read back
+  // between levels, so a buffer must not be replaced under queued scans
+  {
+    int64_t widest = nc;
+    for (int64_t sp : level_split) widest = std::max(widest, 2 * sp);
+    tb = 0;
+    FLK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, d_flag.as<int32_t>(), d_scan.as<int32_t>(),
+                                             (int)widest, s));
+    FLK_HIP(hipStreamSynchronize(s));
+    d_tmp.reserve(std::max<size_t>(tb, 16));
+  }
   int32_t total = nc;
-  for (int lev = 0;; ++lev) {
+  for (size_t lev = 0; lev < level_split.size(); ++lev) {
     const int32_t base = level_base[lev], n = level_n[lev];
+    const int32_t nsplit = (int32_t)level_split[lev];
     const unsigned nb = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(bh_split_flags, dim3(nb), dim3(256), 0, s, d_range.as<BhRange>() + base, n,
                        leaf, d_flag.as<int32_t>());
-    tb = 0;
-    FLK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, d_flag.as<int32_t>(), d_scan.as<int32_t>(),
-                                             n + 1, s));
-    d_tmp.reserve(std::max<size_t>(tb, 16));
-    // a trailing zero flag makes scan[n] the number of split nodes
-    FLK_HIP(hipMemsetAsync(d_flag.as<int32_t>() + n, 0, sizeof(int32_t), s));
-    FLK_HIP(hipcub::DeviceScan::ExclusiveSum(d_tmp.ptr, tb, d_flag.as<int32_t>(), d_scan.as<int32_t>(),
-                                             n + 1, s));
-    int32_t nsplit = 0;
-    FLK_HIP(hipMemcpyAsync(&nsplit, d_scan.as<int32_t>() + n, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    FLK_HIP(hipStreamSynchronize(s));
+    size_t tbl = 0;
+    FLK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tbl, d_flag.as<int32_t>(), d_scan.as<int32_t>(), n, s));
+    if (tbl > d_tmp.bytes) throw Error(FLK_ERR_STATE, "scan scratch too small");
+    FLK_HIP(hipcub::DeviceScan::ExclusiveSum(d_tmp.ptr, tbl, d_flag.as<int32_t>(), d_scan.as<int32_t>(), n, s));
     hipLaunchKernelGGL(bh_split, dim3(nb), dim3(256), 0, s, d_range.as<BhRange>() + base, n, base,
                        total, d_flag.as<int32_t>(), d_scan.as<int32_t>(),
                        d_range.as<BhRange>() + total, d_child.as<int2>());
@@ -873,7 +915,7 @@ void build_forest(flk_ctx *ctx, const std::vector<int32_t> &curves_in, int leaf,
   for (int lev = (int)level_base.size() - 1; lev >= 0; --lev) {
     const int32_t n = level_n[lev];
     hipLaunchKernelGGL(bh_moments, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, d_range.as<BhRange>(),
-                       n, level_base[lev], leaf, d_child.as<int2>(), F.order.as<uint32_t>(),
+                       n, level_base[lev], std::max(leaf, kDirectSegments), d_child.as<int2>(), F.order.as<uint32_t>(),
                        cv.dpts.as<double4>(), d_m64.as<Mom64>(), F.geo.as<BhGeo>(), F.mom.as<BhMom>());
     FLK_HIP(hipGetLastError());
   }
