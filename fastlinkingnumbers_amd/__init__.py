@@ -89,7 +89,8 @@ class FlkTune(C.Structure):
         ("ds_blocks_per_cu", C.c_int32),
         ("ds_tasks_per_wave", C.c_int32),
         ("bh_items_per_thread", C.c_int32),
-        ("reserved", C.c_int32 * 4),
+        ("ingest_threads", C.c_int32),
+        ("reserved", C.c_int32 * 3),
     ]
 
     def as_dict(self) -> dict:
@@ -121,6 +122,7 @@ FLK_EXPORTS = [
     ("flk_last_error", C.c_char_p, [C.c_void_p]),
     ("flk_device_name", C.c_int32, [C.c_void_p, C.c_char_p, C.c_int32]),
     ("flk_set_curves", C.c_int32, [C.c_void_p, _P(C.c_double), _P(C.c_int64), C.c_int32]),
+    ("flk_set_curves_f32", C.c_int32, [C.c_void_p, _P(C.c_float), _P(C.c_int64), C.c_int32]),
     ("flk_curve_boxes", C.c_int32, [C.c_void_p, _P(C.c_double)]),
     ("flk_potential_links", C.c_int32,
      [C.c_void_p, _P(C.c_double), C.c_int32, _P(_P(C.c_int32)), _P(C.c_int64)]),
@@ -253,11 +255,19 @@ class Engine:
         self.set_curves_packed(xyz, offs)
 
     def set_curves_packed(self, xyz: np.ndarray, offs: np.ndarray):
-        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        """Uploads packed curves: xyz (total, 3) and offsets (ncurves + 1).
+        float32 input goes through flk_set_curves_f32, anything else as
+        float64 through flk_set_curves."""
         offs = np.ascontiguousarray(offs, dtype=np.int64)
         n = len(offs) - 1
-        self._check(self.lib.flk_set_curves(self.h, _ptr(xyz, C.c_double), _ptr(offs, C.c_int64), n),
-                    "flk_set_curves")
+        if np.asarray(xyz).dtype == np.float32:
+            xyz = np.ascontiguousarray(xyz, dtype=np.float32)
+            self._check(self.lib.flk_set_curves_f32(self.h, _ptr(xyz, C.c_float), _ptr(offs, C.c_int64), n),
+                        "flk_set_curves_f32")
+        else:
+            xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+            self._check(self.lib.flk_set_curves(self.h, _ptr(xyz, C.c_double), _ptr(offs, C.c_int64), n),
+                        "flk_set_curves")
         self.ncurves = n
         self.sizes = np.diff(offs)
 
@@ -303,7 +313,8 @@ class Engine:
     def discretize(self, curves, types, looped, pairs):
         """Device discretization of control polygons `curves` (segment types
         SEG_*, looped flags) against potentially linked `pairs`; returns the
-        polylines and the counts (rounds, overlaps per round, pieces)."""
+        polylines and the counts (rounds, overlaps per round, pieces).  The
+        polylines also become the engine's curve set."""
         n = len(curves)
         types = np.broadcast_to(np.asarray(types, dtype=np.int32), (n,)).copy()
         looped = np.broadcast_to(np.asarray(looped, dtype=np.uint8), (n,)).copy()
@@ -322,6 +333,8 @@ class Engine:
         self._check(rc, "flk_discretize")
         off = np.ctypeslib.as_array(ooff, shape=(n + 1,)).copy()
         v = np.ctypeslib.as_array(out, shape=(3 * int(off[-1]),)).copy().reshape(-1, 3) if off[-1] else np.zeros((0, 3))
+        self.ncurves = n
+        self.sizes = np.diff(off)
         return [v[off[c]:off[c + 1]] for c in range(n)], d
 
     # ---- launch shapes

@@ -397,8 +397,11 @@ unsigned blocks_for(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + 255)
 
 }  // namespace
 
+// The output polylines become the context's curve set (device to device);
+// res.xyz receives a host copy when host_copy is set.
 void discretize_run(flk_ctx *ctx, const double *ctrl, const int64_t *offsets, const int32_t *types,
-                    const uint8_t *looped, int32_t nc, const int32_t *pairs, int64_t npairs, DiscResult &res) {
+                    const uint8_t *looped, int32_t nc, const int32_t *pairs, int64_t npairs, DiscResult &res,
+                    bool host_copy) {
   hipStream_t s = ctx->stream;
   res = DiscResult{};
   // curves and their segments
@@ -426,7 +429,16 @@ void discretize_run(flk_ctx *ctx, const double *ctrl, const int64_t *offsets, co
   }
   res.segments = nseg;
   res.out_offsets.assign(nc + 1, 0);
-  if (nseg == 0) return;
+  if (nseg == 0) {
+    try {
+      curves_layout(ctx, res.out_offsets.data(), nc);
+      curves_finish(ctx);
+    } catch (...) {
+      curves_clear(ctx);
+      throw;
+    }
+    return;
+  }
   const double eps = std::numeric_limits<double>::epsilon();
   const double threshold = eps * (coord_sum / (nseg * 1.73205080757));
   // potential links as sorted (min << 32 | max) keys
@@ -446,7 +458,12 @@ void discretize_run(flk_ctx *ctx, const double *ctrl, const int64_t *offsets, co
   d_cs.reserve(sizeof(DCurve) * nc);
   d_links.reserve(sizeof(uint64_t) * std::max<size_t>(lk.size(), 1));
   d_cnt.reserve(sizeof(unsigned long long) * 2);
-  FLK_HIP(hipMemcpyAsync(d_q.ptr, ctrl, sizeof(double) * 3 * nctrl, hipMemcpyHostToDevice, s));
+  {
+    DevBuf land;
+    land.reserve(sizeof(float) * 3 * std::max<int64_t>(nctrl, 1));
+    ingest_points(ctx, ctrl, false, nctrl, d_q.as<double>(), land.as<float>(), land.bytes);
+    FLK_HIP(hipStreamSynchronize(s));  // before the landing area is freed
+  }
   FLK_HIP(hipMemcpyAsync(d_cs.ptr, cs.data(), sizeof(DCurve) * nc, hipMemcpyHostToDevice, s));
   if (!lk.empty())
     FLK_HIP(hipMemcpyAsync(d_links.ptr, lk.data(), sizeof(uint64_t) * lk.size(), hipMemcpyHostToDevice, s));
@@ -634,9 +651,21 @@ void discretize_run(flk_ctx *ctx, const double *ctrl, const int64_t *offsets, co
                      ret.c.as<int32_t>(), ret.s.as<int32_t>(), ret.lo.as<double>(), ret.hi.as<double>(), nret,
                      d_ob.as<int32_t>(), d_open.as<int32_t>(), d_out.as<double>());
   FLK_HIP(hipGetLastError());
-  res.xyz.resize(3 * nout);
-  FLK_HIP(hipMemcpyAsync(res.xyz.data(), d_out.ptr, sizeof(double) * 3 * nout, hipMemcpyDeviceToHost, s));
+  if (host_copy) {
+    res.xyz.resize(3 * nout);
+    FLK_HIP(hipMemcpyAsync(res.xyz.data(), d_out.ptr, sizeof(double) * 3 * nout, hipMemcpyDeviceToHost, s));
+  }
   FLK_HIP(hipStreamSynchronize(s));
+  // install the polylines: the output array becomes the curve set's input
+  // array (it holds 3 nout doubles, so the layout's reserve keeps it)
+  std::swap(ctx->curves.raw, d_out);
+  try {
+    curves_layout(ctx, res.out_offsets.data(), nc);
+    curves_finish(ctx);
+  } catch (...) {
+    curves_clear(ctx);
+    throw;
+  }
 }
 
 }  // namespace flk

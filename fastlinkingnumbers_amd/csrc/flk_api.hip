@@ -32,6 +32,103 @@ static thread_local std::string g_create_err;
     return FLK_ERR_STATE;                                    \
   }
 
+namespace flk {
+
+int64_t curves_layout(flk_ctx *ctx, const int64_t *offsets, int32_t ncurves) {
+  Curves &cv = ctx->curves;
+  if (ncurves > 0 && offsets[0] != 0) throw Error(FLK_ERR_ARG, "offsets[0] must be 0");
+  for (int32_t c = 0; c < ncurves; ++c)
+    if (offsets[c + 1] < offsets[c]) throw Error(FLK_ERR_ARG, "offsets must be non-decreasing");
+  for (int32_t c = 0; c < ncurves; ++c)
+    if (offsets[c + 1] - offsets[c] > INT32_MAX - 1)
+      throw Error(FLK_ERR_ARG, "a curve has more than 2^31-2 vertices");
+  const int64_t nv = ncurves ? offsets[ncurves] : 0;
+  cv.ncurves = ncurves;
+  if (ncurves == 0) cv.in_off.assign(1, 0);
+  else cv.in_off.assign(offsets, offsets + ncurves + 1);
+  cv.n.resize(ncurves);
+  cv.off.resize(ncurves + 1);
+  for (int32_t c = 0; c < ncurves; ++c) {
+    cv.n[c] = (int32_t)(offsets[c + 1] - offsets[c]);
+    cv.off[c] = offsets[c] + c;  // one closing vertex per curve
+  }
+  cv.off[ncurves] = nv + ncurves;
+  cv.nvert_dev = nv + ncurves;
+  const int64_t nd = std::max<int64_t>(cv.nvert_dev, 1);
+  cv.raw.reserve(sizeof(double) * 3 * std::max<int64_t>(nv, 1));
+  cv.d_off.reserve(sizeof(int64_t) * (ncurves + 1));
+  cv.dpts.reserve(sizeof(double4) * nd);
+  cv.hi.reserve(sizeof(float4) * nd);  // also the FP32 landing area of the upload
+  cv.lo.reserve(sizeof(float4) * nd);
+  cv.ed.reserve(sizeof(float4) * nd);
+  cv.inexact = false;
+  cv.bh_budget = 0;
+  ++ctx->curves_version;
+  for (auto &g : ctx->ds_graph) g.reset();
+  return nv;
+}
+
+void curves_clear(flk_ctx *ctx) {
+  Curves &cv = ctx->curves;
+  cv.ncurves = 0;
+  cv.in_off.assign(1, 0);
+  cv.off.assign(1, 0);
+  cv.n.clear();
+  cv.box.clear();
+  cv.nvert_dev = 0;
+  cv.inexact = false;
+  cv.bh_budget = 0;
+  ++ctx->curves_version;
+  for (auto &g : ctx->ds_graph) g.reset();
+}
+
+void curves_finish(flk_ctx *ctx) {
+  Curves &cv = ctx->curves;
+  const int32_t ncurves = cv.ncurves;
+  hipStream_t s = ctx->stream;
+  curve_boxes_run(ctx, cv.box);
+  // Origin and power-of-two scale: stored = (x - origin) * scale with the
+  // model extent near 1.  The origin stays 0 unless the model is far from it
+  // (then it is snapped to a coarse grid so fp32 inputs remain exact).
+  double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+  for (int32_t c = 0; c < ncurves; ++c)
+    if (cv.box[6 * c] <= cv.box[6 * c + 3])
+      for (int d = 0; d < 3; ++d) {
+        lo[d] = std::min(lo[d], cv.box[6 * c + d]);
+        hi[d] = std::max(hi[d], cv.box[6 * c + 3 + d]);
+      }
+  double extent = 0.0, far = 0.0;
+  for (int d = 0; d < 3; ++d)
+    if (lo[d] <= hi[d]) {
+      extent = std::max(extent, hi[d] - lo[d]);
+      far = std::max(far, std::fabs(0.5 * (lo[d] + hi[d])));
+    }
+  cv.origin[0] = cv.origin[1] = cv.origin[2] = 0.0;
+  if (extent > 0 && far > 4 * extent) {
+    const double grid = std::ldexp(1.0, (int)std::floor(std::log2(extent)));
+    for (int d = 0; d < 3; ++d) cv.origin[d] = grid * std::round(0.5 * (lo[d] + hi[d]) / grid);
+  }
+  double span = 0.0;
+  for (int d = 0; d < 3; ++d)
+    if (lo[d] <= hi[d])
+      span = std::max(span, std::max(std::fabs(hi[d] - cv.origin[d]), std::fabs(lo[d] - cv.origin[d])));
+  cv.scale = span > 0 ? std::ldexp(1.0, -(int)std::ceil(std::log2(span))) : 1.0;
+  if (!std::isfinite(cv.scale) || cv.scale == 0) cv.scale = 1.0;
+  if (cv.nvert_dev > 0) {
+    stage_vertices_launch(ctx);
+    ctx->d_count.reserve(sizeof(int32_t));
+    FLK_HIP(hipMemsetAsync(ctx->d_count.ptr, 0, sizeof(int32_t), s));
+    convert_vertices_launch(s, cv.dpts.as<double4>(), cv.nvert_dev, cv.hi.as<float4>(), cv.lo.as<float4>(),
+                            cv.ed.as<float4>(), ctx->d_count.as<int32_t>());
+    int32_t flag = 0;
+    FLK_HIP(hipMemcpyAsync(&flag, ctx->d_count.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    FLK_HIP(hipStreamSynchronize(s));
+    cv.inexact = flag != 0;
+  }
+}
+
+}  // namespace flk
+
 extern "C" {
 
 int32_t flk_default_params(flk_params *p) {
@@ -96,6 +193,7 @@ void flk_destroy(flk_ctx *ctx) {
                          &ctx->mlp.d_params, &ctx->mlp.d_norm, &ctx->mlp.d_bf16};
   for (auto *b : bufs) b->release();
   ctx->bh.release();
+  ctx->stage.release();
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   if (ctx->ev2) (void)hipEventDestroy(ctx->ev2);
@@ -118,77 +216,33 @@ int32_t flk_set_curves(flk_ctx *ctx, const double *xyz, const int64_t *offsets, 
   if (!ctx || ncurves < 0 || (ncurves > 0 && !offsets)) return FLK_ERR_ARG;
   FLK_GUARD(ctx, {
     FLK_HIP(hipSetDevice(ctx->device));
-    flk::Curves &cv = ctx->curves;
-    if (offsets && offsets[0] != 0) throw Error(FLK_ERR_ARG, "offsets[0] must be 0");
-    for (int32_t c = 0; c < ncurves; ++c)
-      if (offsets[c + 1] < offsets[c]) throw Error(FLK_ERR_ARG, "offsets must be non-decreasing");
-    for (int32_t c = 0; c < ncurves; ++c)
-      if (offsets[c + 1] - offsets[c] > INT32_MAX - 1)
-        throw Error(FLK_ERR_ARG, "a curve has more than 2^31-2 vertices");
-    const int64_t nv = ncurves ? offsets[ncurves] : 0;
-    if (nv > 0 && !xyz) throw Error(FLK_ERR_ARG, "xyz is NULL");
-    cv.ncurves = ncurves;
-    cv.in_off.assign(offsets, offsets + ncurves + 1);
-    if (ncurves == 0) cv.in_off.assign(1, 0);
-    cv.n.resize(ncurves);
-    cv.off.resize(ncurves + 1);
-    for (int32_t c = 0; c < ncurves; ++c) {
-      cv.n[c] = (int32_t)(offsets[c + 1] - offsets[c]);
-      cv.off[c] = offsets[c] + c;  // one closing vertex per curve
+    if (ncurves > 0 && offsets[ncurves] > 0 && !xyz) throw Error(FLK_ERR_ARG, "xyz is NULL");
+    try {
+      const int64_t nv = flk::curves_layout(ctx, offsets, ncurves);
+      flk::Curves &cv = ctx->curves;
+      flk::ingest_points(ctx, xyz, false, nv, cv.raw.as<double>(), cv.hi.as<float>(), cv.hi.bytes);
+      flk::curves_finish(ctx);
+    } catch (...) {
+      flk::curves_clear(ctx);  // never leave a half-built curve set behind
+      throw;
     }
-    cv.off[ncurves] = nv + ncurves;
-    cv.nvert_dev = nv + ncurves;
-    hipStream_t s = ctx->stream;
-    cv.raw.reserve(sizeof(double) * 3 * std::max<int64_t>(nv, 1));
-    if (nv) FLK_HIP(hipMemcpyAsync(cv.raw.ptr, xyz, sizeof(double) * 3 * nv, hipMemcpyHostToDevice, s));
-    flk::curve_boxes_run(ctx, cv.box);
-    // Origin and power-of-two scale: stored = (x - origin) * scale with the
-    // model extent near 1.  The origin stays 0 unless the model is far from it
-    // (then it is snapped to a coarse grid so fp32 inputs remain exact).
-    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
-    for (int32_t c = 0; c < ncurves; ++c)
-      if (cv.box[6 * c] <= cv.box[6 * c + 3])
-        for (int d = 0; d < 3; ++d) {
-          lo[d] = std::min(lo[d], cv.box[6 * c + d]);
-          hi[d] = std::max(hi[d], cv.box[6 * c + 3 + d]);
-        }
-    double extent = 0.0, far = 0.0;
-    for (int d = 0; d < 3; ++d)
-      if (lo[d] <= hi[d]) {
-        extent = std::max(extent, hi[d] - lo[d]);
-        far = std::max(far, std::fabs(0.5 * (lo[d] + hi[d])));
-      }
-    cv.origin[0] = cv.origin[1] = cv.origin[2] = 0.0;
-    if (extent > 0 && far > 4 * extent) {
-      const double grid = std::ldexp(1.0, (int)std::floor(std::log2(extent)));
-      for (int d = 0; d < 3; ++d) cv.origin[d] = grid * std::round(0.5 * (lo[d] + hi[d]) / grid);
-    }
-    double span = 0.0;
-    for (int d = 0; d < 3; ++d)
-      if (lo[d] <= hi[d])
-        span = std::max(span, std::max(std::fabs(hi[d] - cv.origin[d]), std::fabs(lo[d] - cv.origin[d])));
-    cv.scale = span > 0 ? std::ldexp(1.0, -(int)std::ceil(std::log2(span))) : 1.0;
-    if (!std::isfinite(cv.scale) || cv.scale == 0) cv.scale = 1.0;
-    cv.d_off.reserve(sizeof(int64_t) * (ncurves + 1));
-    cv.dpts.reserve(sizeof(double4) * std::max<int64_t>(cv.nvert_dev, 1));
-    cv.hi.reserve(sizeof(float4) * std::max<int64_t>(cv.nvert_dev, 1));
-    cv.lo.reserve(sizeof(float4) * std::max<int64_t>(cv.nvert_dev, 1));
-    cv.ed.reserve(sizeof(float4) * std::max<int64_t>(cv.nvert_dev, 1));
-    cv.inexact = false;
-    cv.bh_budget = 0;
-    ++ctx->curves_version;
-    for (auto &g : ctx->ds_graph) g.reset();
-    if (cv.nvert_dev > 0) {
-      flk::stage_vertices_launch(ctx);
-      ctx->d_count.reserve(sizeof(int32_t));
-      FLK_HIP(hipMemsetAsync(ctx->d_count.ptr, 0, sizeof(int32_t), s));
-      flk::convert_vertices_launch(s, cv.dpts.as<double4>(), cv.nvert_dev, cv.hi.as<float4>(),
-                                   cv.lo.as<float4>(), cv.ed.as<float4>(),
-                                   ctx->d_count.as<int32_t>());
-      int32_t flag = 0;
-      FLK_HIP(hipMemcpyAsync(&flag, ctx->d_count.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-      FLK_HIP(hipStreamSynchronize(s));
-      cv.inexact = flag != 0;
+    return FLK_OK;
+  })
+}
+
+int32_t flk_set_curves_f32(flk_ctx *ctx, const float *xyz, const int64_t *offsets, int32_t ncurves) {
+  if (!ctx || ncurves < 0 || (ncurves > 0 && !offsets)) return FLK_ERR_ARG;
+  FLK_GUARD(ctx, {
+    FLK_HIP(hipSetDevice(ctx->device));
+    if (ncurves > 0 && offsets[ncurves] > 0 && !xyz) throw Error(FLK_ERR_ARG, "xyz is NULL");
+    try {
+      const int64_t nv = flk::curves_layout(ctx, offsets, ncurves);
+      flk::Curves &cv = ctx->curves;
+      flk::ingest_points(ctx, xyz, true, nv, cv.raw.as<double>(), cv.hi.as<float>(), cv.hi.bytes);
+      flk::curves_finish(ctx);
+    } catch (...) {
+      flk::curves_clear(ctx);  // never leave a half-built curve set behind
+      throw;
     }
     return FLK_OK;
   })
@@ -296,7 +350,7 @@ int32_t flk_discretize(flk_ctx *ctx, const double *ctrl_xyz, const int64_t *offs
                        const int32_t *segment_types, const uint8_t *looped, int32_t ncurves,
                        const int32_t *pairs, int64_t npairs, const double **out_xyz,
                        const int64_t **out_offsets, flk_disc_info *info) {
-  if (!ctx || ncurves < 0 || npairs < 0 || !out_xyz || !out_offsets) return FLK_ERR_ARG;
+  if (!ctx || ncurves < 0 || npairs < 0) return FLK_ERR_ARG;
   if (ncurves > 0 && (!offsets || !segment_types || !looped)) return FLK_ERR_ARG;
   if (npairs > 0 && !pairs) return FLK_ERR_ARG;
   auto fill = [&]() {
@@ -316,7 +370,8 @@ int32_t flk_discretize(flk_ctx *ctx, const double *ctrl_xyz, const int64_t *offs
     FLK_HIP(hipSetDevice(ctx->device));
     if (ncurves > 0 && offsets[0] != 0) throw Error(FLK_ERR_ARG, "offsets[0] must be 0");
     if (ncurves > 0 && offsets[ncurves] > 0 && !ctrl_xyz) throw Error(FLK_ERR_ARG, "ctrl_xyz is NULL");
-    flk::discretize_run(ctx, ctrl_xyz, offsets, segment_types, looped, ncurves, pairs, npairs, ctx->disc);
+    flk::discretize_run(ctx, ctrl_xyz, offsets, segment_types, looped, ncurves, pairs, npairs, ctx->disc,
+                        out_xyz != nullptr);
   } catch (const flk::Error &e) {
     ctx->disc.rounds = (int32_t)ctx->disc.overlaps.size();
     fill();
@@ -330,8 +385,8 @@ int32_t flk_discretize(flk_ctx *ctx, const double *ctrl_xyz, const int64_t *offs
     return FLK_ERR_STATE;
   }
   fill();
-  *out_xyz = ctx->disc.xyz.data();
-  *out_offsets = ctx->disc.out_offsets.data();
+  if (out_xyz) *out_xyz = ctx->disc.xyz.data();
+  if (out_offsets) *out_offsets = ctx->disc.out_offsets.data();
   return FLK_OK;
 }
 
@@ -352,6 +407,8 @@ int32_t flk_set_tune(flk_ctx *ctx, const flk_tune *tune) {
       throw Error(FLK_ERR_ARG, "ds_tasks_per_wave must be in [1, 1024]");
     if (tune->bh_items_per_thread != 2 && tune->bh_items_per_thread != 4 && tune->bh_items_per_thread != 8)
       throw Error(FLK_ERR_ARG, "bh_items_per_thread must be 2, 4 or 8");
+    if (tune->ingest_threads < 0 || tune->ingest_threads > 64)
+      throw Error(FLK_ERR_ARG, "ingest_threads must be in [0, 64]");
     ctx->tune = *tune;
   })
   return FLK_OK;

@@ -110,10 +110,8 @@ struct Curves {
   std::vector<int64_t> off;      // host offsets into the device vertex arrays (n+1 per curve)
   std::vector<int32_t> n;        // vertices per curve
   std::vector<double> box;       // ncurves x 6, FP64, in input coordinates
-  std::vector<uint8_t> exact;    // curve coordinates exactly representable in FP32
   double origin[3] = {0, 0, 0};  // input = origin + scale^-1 * stored
   double scale = 1.0;            // power of two
-  std::vector<double> xyz;       // host copy of the input (FP64 AoS)
   std::vector<int64_t> in_off;   // input offsets
   DevBuf raw;                    // input FP64 AoS (no closing vertex)
   DevBuf dpts;                   // double4 per vertex, shifted+scaled, +1 closing vertex per curve
@@ -122,6 +120,17 @@ struct Curves {
   int64_t nvert_dev = 0;
   bool inexact = false;          // some stored coordinate needs the FP32 lo part
   int64_t bh_budget = 0;         // Barnes–Hut batch size in segments, fixed per curve set
+};
+
+// Pinned host staging buffers of the coordinate upload, two per worker
+// thread, each with an event marking the end of its last DMA.
+struct HostStage {
+  static constexpr size_t kChunkBytes = size_t(8) << 20;
+  std::vector<void *> buf;
+  std::vector<hipEvent_t> done;
+  std::vector<uint8_t> pending;  // a DMA from buf[i] may still run
+  void ensure(int workers);
+  void release();
 };
 
 // Barnes–Hut device scratch, kept by the context between calls (grow-only)
@@ -182,7 +191,7 @@ struct DiscResult {
   int32_t rounds = 0;
   std::vector<int64_t> overlaps;     // overlapping piece pairs per round
   std::vector<int64_t> out_offsets;  // ncurves + 1, into xyz / 3
-  std::vector<double> xyz;           // polyline vertices, AoS
+  std::vector<double> xyz;           // polyline vertices, AoS (when copied to the host)
   int32_t small_curve = -1, small_segment = -1;  // the input guard's smallest segment
   double small_size = 0.0;
 };
@@ -208,9 +217,10 @@ struct flk_ctx {
   std::vector<int32_t> pls_pairs;
   flk::MlpState mlp;
   flk::BhScratch bh;
+  flk::HostStage stage;
   // default shapes: measured best on the 1,600-loop chain mail (R = 4 with
   // every block resident: 985 ms against 1,016 ms for R = 8)
-  flk_tune tune{4, 0, 8, 4, {0, 0, 0, 0}};
+  flk_tune tune{4, 0, 8, 4, 0, {0, 0, 0}};
   flk::DsGraph ds_graph[2];      // FP32, FP64 batches
   flk::DiscResult disc;          // last flk_discretize result
   uint64_t curves_version = 0;   // bumped by every flk_set_curves
@@ -235,9 +245,11 @@ float ds_time_sample(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, const f
 int64_t ds_work(flk_ctx *ctx, const int32_t *pairs, int64_t npairs);
 
 // Spline-to-polyline discretization of the given curves against the given
-// potentially linked pairs (see disc.hip).
+// potentially linked pairs (see disc.hip); the polylines become the context's
+// curve set, and res.xyz a host copy of them when host_copy is set.
 void discretize_run(flk_ctx *ctx, const double *ctrl, const int64_t *offsets, const int32_t *types,
-                    const uint8_t *looped, int32_t nc, const int32_t *pairs, int64_t npairs, DiscResult &res);
+                    const uint8_t *looped, int32_t nc, const int32_t *pairs, int64_t npairs, DiscResult &res,
+                    bool host_copy);
 
 void bh_run(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, const flk_params &prm,
             std::vector<double> &lk, std::vector<double> &err, std::vector<uint32_t> &flags);
@@ -245,6 +257,23 @@ void bh_run(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, const flk_params
 void pls_run(flk_ctx *ctx, const double *boxes, int32_t ncurves, std::vector<int32_t> &out);
 
 void curve_boxes_run(flk_ctx *ctx, std::vector<double> &boxes);
+
+// A new curve set: validates the offsets and sizes the device arrays; the
+// caller then fills curves.raw (input FP64 AoS) and calls curves_finish, which
+// computes the boxes, origin and scale and the staged vertex arrays.  Returns
+// the vertex count.
+int64_t curves_layout(flk_ctx *ctx, const int64_t *offsets, int32_t ncurves);
+void curves_finish(flk_ctx *ctx);
+// Resets to the empty curve set (after a failure part-way through a new one).
+void curves_clear(flk_ctx *ctx);
+
+// Copies n host xyz triples (FP32, or FP64 sent as FP32 when every value fits
+// exactly) to dst (device FP64 AoS) through the pinned staging buffers; land
+// is device scratch for the FP32 copy (>= 12 n bytes).  Enqueued on the
+// context's stream; the host array is no longer read on return.  Returns the
+// bytes sent.
+int64_t ingest_points(flk_ctx *ctx, const void *src, bool src_f32, int64_t n, double *dst, float *land,
+                      size_t land_bytes);
 void convert_vertices_launch(hipStream_t s, const double4 *d, int64_t n, float4 *hi, float4 *lo,
                              float4 *ed, int32_t *inexact);
 

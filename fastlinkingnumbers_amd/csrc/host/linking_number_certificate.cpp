@@ -93,16 +93,17 @@ void upload(flk_ctx *ctx, const std::vector<Curve> &curves) {
                "flk_set_curves");
 }
 
-std::vector<std::pair<int, int>> potential_links(flk_ctx *ctx, const std::vector<Curve> &curves,
-                                                 bool device_boxes) {
-  const int32_t n = (int32_t)curves.size();
+// Potential links of the curves (their exact boxes computed here), or, with
+// curves == nullptr, of the engine's current n curves (device boxes).
+std::vector<std::pair<int, int>> potential_links(flk_ctx *ctx, const std::vector<Curve> *curves, int32_t n) {
+  const bool device_boxes = curves == nullptr;
   std::vector<double> boxes;
   if (!device_boxes) {
     boxes.resize(6 * (size_t)n);
 #pragma omp parallel for schedule(dynamic)
     for (int32_t i = 0; i < n; ++i) {
-      curves[i].ComputeCurveBoundingBox();
-      const Box3d b = curves[i].get_last_curve_bounding_box();
+      (*curves)[i].ComputeCurveBoundingBox();
+      const Box3d b = (*curves)[i].get_last_curve_bounding_box();
       for (int d = 0; d < 3; ++d) {
         boxes[6 * i + d] = b.min()[d];
         boxes[6 * i + 3 + d] = b.max()[d];
@@ -119,10 +120,11 @@ std::vector<std::pair<int, int>> potential_links(flk_ctx *ctx, const std::vector
 }
 
 // Discretize (reference src/discretize.cpp:102-281) on the device: the
-// engine bisects the pieces and evaluates the polyline vertices; this prints
-// the reference's progress and failure messages and builds the curves.
-std::vector<Curve> discretize_curves(flk_ctx *ctx, const std::vector<Curve> &in,
-                                     const std::vector<std::pair<int, int>> &links, bool verbose) {
+// engine bisects the pieces, evaluates the polyline vertices and keeps them as
+// its curve set; this prints the reference's progress and failure messages
+// and returns the looped flag of each output polyline.
+std::vector<uint8_t> discretize_curves(flk_ctx *ctx, const std::vector<Curve> &in,
+                                       const std::vector<std::pair<int, int>> &links, bool verbose) {
   const int32_t nc = (int32_t)in.size();
   std::vector<int64_t> off(nc + 1, 0);
   std::vector<int32_t> types(nc);
@@ -133,19 +135,20 @@ std::vector<Curve> discretize_curves(flk_ctx *ctx, const std::vector<Curve> &in,
     looped[c] = in[c].is_looped() ? 1 : 0;
   }
   std::vector<double> ctrl(3 * (size_t)off[nc]);
+#pragma omp parallel for schedule(dynamic)
   for (int32_t c = 0; c < nc; ++c)
-    for (size_t k = 0; k < in[c].get_q().size(); ++k)
-      for (int d = 0; d < 3; ++d) ctrl[3 * (off[c] + (int64_t)k) + d] = in[c].get_q()[k][d];
+    if (!in[c].get_q().empty())
+      std::copy(in[c].get_q()[0].data(), in[c].get_q()[0].data() + 3 * in[c].get_q().size(),
+                ctrl.data() + 3 * off[c]);
   std::vector<int32_t> pairs(2 * links.size());
   for (size_t k = 0; k < links.size(); ++k) {
     pairs[2 * k] = links[k].first;
     pairs[2 * k + 1] = links[k].second;
   }
-  const double *xyz = nullptr;
   const int64_t *voff = nullptr;
   flk_disc_info info;
   const int32_t rc = flk_discretize(ctx, ctrl.data(), off.data(), types.data(), looped.data(), nc, pairs.data(),
-                                    (int64_t)links.size(), &xyz, &voff, &info);
+                                    (int64_t)links.size(), nullptr, &voff, &info);
   if (verbose)
     for (int32_t r = 0; r < info.rounds && r < 55; ++r)
       if (rc == FLK_OK || r + 1 < info.rounds || info.small_curve >= 0 || info.rounds == 55)
@@ -171,18 +174,9 @@ std::vector<Curve> discretize_curves(flk_ctx *ctx, const std::vector<Curve> &in,
     std::cout << "Original segment count: " << info.segments << ", discretized count: " << info.pieces
               << std::endl;
   }
-  std::vector<Curve> out(nc);
-  for (int32_t c = 0; c < nc; ++c) {
-    Curve &o = out[c];
-    o.set_segment_type(SegmentType::Polyline);
-    const int64_t n = voff[c + 1] - voff[c];
-    o.get_q().resize((size_t)n);
-    for (int64_t k = 0; k < n; ++k)
-      o.get_q()[(size_t)k] = Point3(xyz[3 * (voff[c] + k)], xyz[3 * (voff[c] + k) + 1], xyz[3 * (voff[c] + k) + 2]);
-    // an open input curve with pieces stays open (its last end point closes the list)
-    o.set_looped(in[c].is_looped() || n == 0);
-    o.ComputeCurveBoundingBox();
-  }
+  // an open input curve with pieces stays open (its last end point closes the list)
+  std::vector<uint8_t> out(nc);
+  for (int32_t c = 0; c < nc; ++c) out[c] = (in[c].is_looped() || voff[c + 1] == voff[c]) ? 1 : 0;
   return out;
 }
 
@@ -226,25 +220,23 @@ void LinkingNumberCertificate::ComputeFromModel(const Model &model, bool force_d
   flk_ctx *ctx = engine(opt.device);
   const std::vector<Curve> &in = model.GetCurves();
   const int nc = (int)in.size();
-  std::vector<Curve> disc;
-  const std::vector<Curve> *curves = &in;
   std::vector<std::pair<int, int>> pairs;
+  std::vector<uint8_t> looped(nc);
   bool all_pl = true;
   for (const auto &c : in) all_pl = all_pl && c.get_segment_type() == SegmentType::Polyline;
   if (discretize) {
-    auto first = potential_links(ctx, in, false);
+    auto first = potential_links(ctx, &in, nc);
     if (v) std::cout << "Starting Discretization." << std::endl;
-    disc = discretize_curves(ctx, in, first, v);
-    curves = &disc;
-    upload(ctx, disc);
-    pairs = potential_links(ctx, disc, true);
+    looped = discretize_curves(ctx, in, first, v);
+    pairs = potential_links(ctx, nullptr, nc);
   } else {
     upload(ctx, in);
-    pairs = potential_links(ctx, in, all_pl);
+    pairs = potential_links(ctx, all_pl ? nullptr : &in, nc);
+    for (int c = 0; c < nc; ++c) looped[c] = in[c].is_looped() ? 1 : 0;
     if (v) std::cout << "Skipping Discretization." << std::endl;
   }
   for (auto &p : pairs)
-    if (!(*curves)[p.first].is_looped() || !(*curves)[p.second].is_looped())
+    if (!looped[p.first] || !looped[p.second])
       throw std::runtime_error("Curve::GetStartPoints() does not support unlooped curves.");
 
   flk_params prm;

@@ -108,6 +108,15 @@ int32_t flk_device_name(const flk_ctx *ctx, char *buf, int32_t len);
 int32_t flk_set_curves(flk_ctx *ctx, const double *xyz, const int64_t *offsets,
                        int32_t ncurves);
 
+/* flk_set_curves for FP32 input (12 bytes per vertex over PCIe): the layout of
+ * a BCC file's control points, which the reference widens to FP64 Curve
+ * objects on import (src/model.cpp:299-345) before every computation.  The
+ * result is identical to flk_set_curves on the same values widened to FP64;
+ * flk_set_curves itself sends FP64 input as FP32 whenever every value
+ * converts exactly. */
+int32_t flk_set_curves_f32(flk_ctx *ctx, const float *xyz, const int64_t *offsets,
+                           int32_t ncurves);
+
 /* Curve bounding boxes of the uploaded polylines, ncurves x (min xyz, max xyz)
  * in FP64.  Replaces PrecomputeCurveBoundingBoxes,
  * src/potential_link_search.cpp:68-73 (polyline case of curve.cpp:331-343). */
@@ -162,8 +171,11 @@ typedef struct flk_disc_info {
  * boxes) until no overlap is left, then returns each curve as a polyline of
  * its pieces' start points (an open curve also gets its last end point).
  * ctrl_xyz: control points, AoS, curve c = rows offsets[c]..offsets[c+1];
- * segment_types: FLK_SEG_*; looped: 0/1 per curve.  *out_xyz / *out_offsets
- * are context-owned (valid until the next call); *info (may be NULL) gets the
+ * segment_types: FLK_SEG_*; looped: 0/1 per curve.  On success the polylines
+ * become the context's curve set, as after flk_set_curves on them (they stay
+ * on the device: no copy back and up again), and *out_xyz / *out_offsets
+ * receive a context-owned host copy (valid until the next call; either may be
+ * NULL, and out_xyz NULL skips the copy).  *info (may be NULL) gets the
  * counts and, on the reference's failures (smallest input segment below
  * eps * mean |q| / sqrt(3), a new piece below it, 55 rounds with overlaps
  * left), the error details (status FLK_ERR_ARG, message as the reference's). */
@@ -183,7 +195,9 @@ typedef struct flk_tune {
                                   per CU (0: as many as are resident at once) */
   int32_t ds_tasks_per_wave;   /* column chunking: target tasks per wave */
   int32_t bh_items_per_thread; /* Barnes–Hut level and leaf kernels: 2, 4 or 8 */
-  int32_t reserved[4];
+  int32_t ingest_threads;      /* host threads filling the upload's pinned
+                                  staging buffers (0: half the cores, at most 8) */
+  int32_t reserved[3];
 } flk_tune;
 
 int32_t flk_get_tune(const flk_ctx *ctx, flk_tune *out);
