@@ -14,7 +14,14 @@
 // grid of relative vectors G(i, j) = q_j - p_i is shared between the four
 // segment pairs that touch it, so each pair costs one new grid point (one
 // sqrt), one row dot, one column dot, one diagonal dot, the triple product
-// and one arctangent.  Per-lane sums are FP32 over short column runs and FP64
+// and one arctangent.  The row and column dots come from squared norms by
+// polarization, G.G' = (|G|^2 + |G'|^2 - |G' - G|^2) / 2, where G' - G is a
+// segment's edge: -|edge|^2 / 2 is stored with each edge (computed in FP64),
+// so such a dot costs two instructions instead of three.  The diagonal dot
+// and the triple product use only the new column: with a = G(k,j),
+// c = G(k+1,j+1), d = G(k,j+1) = a + e2, a.c = d.c - e2.c and
+// a.(e2 x e1) = d.(e2 x e1), so a step keeps only the norms of the previous
+// column in registers, not its vectors.  Per-lane sums are FP32 over short column runs and FP64
 // beyond; the integer branch corrections are exact.
 //
 // Load balance.  All tasks of all pairs form one list with an integer cost
@@ -104,7 +111,7 @@ __global__ __launch_bounds__(256) void ds_tile_f32(
 
     float px[R + 1], py[R + 1], pz[R + 1];
     float lx[R + 1], ly[R + 1], lz[R + 1];
-    float ex[R], ey[R], ez[R];
+    float ex[R], ey[R], ez[R], he1[R];
 #pragma unroll
     for (int k = 0; k <= R; ++k) {
       const int64_t v = pr.off_r + min(rbase + k, pr.n_r);
@@ -126,11 +133,13 @@ __global__ __launch_bounds__(256) void ds_tile_f32(
       ex[k] = valid ? e.x : 0.f;
       ey[k] = valid ? e.y : 0.f;
       ez[k] = valid ? e.z : 0.f;
+      he1[k] = valid ? e.w : 0.f;  // -|e1|^2 / 2
     }
 
-    // Relative vectors of one grid column: G(k, j) = q_j - p_k and its norm.
+    // Relative vectors of one grid column: G(k, j) = q_j - p_k, its squared
+    // norm and its norm.
     auto column = [&](const float4 &q, const float4 &ql, float (&gx)[R + 1], float (&gy)[R + 1],
-                      float (&gz)[R + 1], float (&gl)[R + 1]) {
+                      float (&gz)[R + 1], float (&gn)[R + 1], float (&gl)[R + 1]) {
 #pragma unroll
       for (int k = 0; k <= R; ++k) {
         if constexpr (PRECISE) {
@@ -142,20 +151,21 @@ __global__ __launch_bounds__(256) void ds_tile_f32(
           gy[k] = q.y - py[k];
           gz[k] = q.z - pz[k];
         }
-        gl[k] = __builtin_amdgcn_sqrtf(fmaf(gx[k], gx[k], fmaf(gy[k], gy[k], gz[k] * gz[k])));
+        gn[k] = fmaf(gx[k], gx[k], fmaf(gy[k], gy[k], gz[k] * gz[k]));
+        gl[k] = __builtin_amdgcn_sqrtf(gn[k]);
       }
     };
     const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
     const int64_t vlast = pr.off_c + pr.n_c;  // closing vertex of the column curve
-    float ax[R + 1], ay[R + 1], az[R + 1], al[R + 1], acol[R];
-    float bx[R + 1], by[R + 1], bz[R + 1], bl[R + 1], bcol[R];
+    float an[R + 1], al[R + 1], acol[R];
+    float bn[R + 1], bl[R + 1], bcol[R];
     {
+      float gx[R + 1], gy[R + 1], gz[R + 1];
       const int64_t v = pr.off_c + tk.col0;
-      column(hi[v], PRECISE ? lo[v] : zero4, ax, ay, az, al);
+      column(hi[v], PRECISE ? lo[v] : zero4, gx, gy, gz, an, al);
     }
 #pragma unroll
-    for (int k = 0; k < R; ++k)
-      acol[k] = fmaf(ax[k], ax[k + 1], fmaf(ay[k], ay[k + 1], az[k] * az[k + 1]));
+    for (int k = 0; k < R; ++k) acol[k] = fmaf(0.5f, an[k] + an[k + 1], he1[k]);
 
     // Column vertex j+1 and edge j of the next step, loaded one step ahead
     // (scalar loads; the index is clamped to the closing vertex).
@@ -165,10 +175,8 @@ __global__ __launch_bounds__(256) void ds_tile_f32(
 
     // One column step: the new column (j+1), then the R segment pairs (k, j)
     // of the quadrilaterals a=G(k,j) b=G(k+1,j) c=G(k+1,j+1) d=G(k,j+1).
-    auto step = [&](int64_t v, const float (&gx)[R + 1], const float (&gy)[R + 1],
-                    const float (&gz)[R + 1], const float (&gl)[R + 1], const float (&gcol)[R],
-                    float (&nx)[R + 1], float (&ny)[R + 1], float (&nz)[R + 1],
-                    float (&nl)[R + 1], float (&ncol)[R], float &accf, int &corr) {
+    auto step = [&](int64_t v, const float (&gn)[R + 1], const float (&gl)[R + 1], const float (&gcol)[R],
+                    float (&nn)[R + 1], float (&nl)[R + 1], float (&ncol)[R], float &accf, int &corr) {
       const float4 q = nq, ql = nql, e2 = ne;
       {
         const int64_t w = min(v + 2, vlast);
@@ -176,14 +184,18 @@ __global__ __launch_bounds__(256) void ds_tile_f32(
         if constexpr (PRECISE) nql = lo[w];
         ne = ed[min(v + 1, vlast)];
       }
-      column(q, ql, nx, ny, nz, nl);
+      float nx[R + 1], ny[R + 1], nz[R + 1];
+      column(q, ql, nx, ny, nz, nn, nl);
       float row[R + 1], diag[R];
+      // G(k,j).G(k,j+1) and G(k,j+1).G(k+1,j+1) by polarization (e2.w, he1:
+      // -|edge|^2 / 2), the diagonal G(k,j).G(k+1,j+1) as G(k,j+1).G(k+1,j+1)
+      // - e2.G(k+1,j+1)
 #pragma unroll
-      for (int k = 0; k <= R; ++k) row[k] = fmaf(gx[k], nx[k], fmaf(gy[k], ny[k], gz[k] * nz[k]));
+      for (int k = 0; k <= R; ++k) row[k] = fmaf(0.5f, gn[k] + nn[k], e2.w);
 #pragma unroll
       for (int k = 0; k < R; ++k) {
-        ncol[k] = fmaf(nx[k], nx[k + 1], fmaf(ny[k], ny[k + 1], nz[k] * nz[k + 1]));
-        diag[k] = fmaf(gx[k], nx[k + 1], fmaf(gy[k], ny[k + 1], gz[k] * nz[k + 1]));
+        ncol[k] = fmaf(0.5f, nn[k] + nn[k + 1], he1[k]);
+        diag[k] = fmaf(-e2.x, nx[k + 1], fmaf(-e2.y, ny[k + 1], fmaf(-e2.z, nz[k + 1], ncol[k])));
       }
       float xp[R], yp[R], s12[R], dt[R];
       bool small = true;
@@ -193,29 +205,43 @@ __global__ __launch_bounds__(256) void ds_tile_f32(
         const float s = fmaf(la, lc, diag[k]);
         const float div1 = fmaf(lb, s, fmaf(gcol[k], lc, row[k + 1] * la));
         const float div2 = fmaf(ld, s, fmaf(row[k], lc, ncol[k] * la));
-        // det = a . (e2 x e1)
+        // det = a . (e2 x e1) = d . (e2 x e1)
         const float cx = fmaf(e2.y, ez[k], -e2.z * ey[k]);
         const float cy = fmaf(e2.z, ex[k], -e2.x * ez[k]);
         const float cz = fmaf(e2.x, ey[k], -e2.y * ex[k]);
-        dt[k] = fmaf(gx[k], cx, fmaf(gy[k], cy, gz[k] * cz));
+        dt[k] = fmaf(nx[k], cx, fmaf(ny[k], cy, nz[k] * cz));
         xp[k] = fmaf(div1, div2, -dt[k] * dt[k]);
         s12[k] = div1 + div2;
         yp[k] = dt[k] * s12[k];
         // small angle: x' > 0, no branch correction, |y'/x'| <= 1/8 (per-pair
         // tests interleave with the arithmetic; a max/min fold at the end of
-        // the step serialises and measured 9% slower)
+        // the step serialises and measured 5-9% slower)
         small = small && (s12[k] > 0.f) && (fmaf(0.125f, xp[k], -fabsf(yp[k])) > 0.f);
       }
-      if (__all(small)) {
-        // atan(t) = t - t^3/3 + t^5/5 - t^7/7 for |t| <= 1/8 (error < 1e-9)
+      // Two small angles add as one: arg(z1) + arg(z2) = arg(z1 z2) with
+      // z = x' + i y', so a couple of pairs needs one reciprocal (the
+      // transcendental unit is the scarce one).  Both |y'/x'| <= 1/8 keep the
+      // product's real part above 63/64 x1 x2 > 0 and its tangent within
+      // 16/63; the product must stay clear of FP32 underflow.
+      float zr[R / 2], zi[R / 2];
 #pragma unroll
-        for (int k = 0; k < R; ++k) {
-          const float tt = yp[k] * __builtin_amdgcn_rcpf(xp[k]);
+      for (int c = 0; c < R / 2; ++c) {
+        zr[c] = fmaf(xp[2 * c], xp[2 * c + 1], -yp[2 * c] * yp[2 * c + 1]);
+        zi[c] = fmaf(xp[2 * c], yp[2 * c + 1], yp[2 * c] * xp[2 * c + 1]);
+        small = small && (zr[c] > 1e-30f);
+      }
+      if (__all(small)) {
+        // atan(t) = t P(t^2) on |t| <= 16/63: minimax coefficients fitted for
+        // this engine (relative error 2e-10 in exact arithmetic, no bias)
+#pragma unroll
+        for (int c = 0; c < R / 2; ++c) {
+          const float tt = zi[c] * __builtin_amdgcn_rcpf(zr[c]);
           const float s2 = tt * tt;
-          float c = fmaf(s2, -0.14285714285714285f, 0.2f);
-          c = fmaf(s2, c, -0.3333333333333333f);
-          c = fmaf(s2, c, 1.0f);
-          accf = fmaf(tt, c, accf);
+          float p = fmaf(s2, 0.09799769520759583f, -0.14216716587543488f);
+          p = fmaf(s2, p, 0.19998548924922943f);
+          p = fmaf(s2, p, -0.33333322405815125f);
+          p = fmaf(s2, p, 1.0f);
+          accf = fmaf(tt, p, accf);
         }
       } else {
 #pragma unroll
@@ -234,14 +260,14 @@ __global__ __launch_bounds__(256) void ds_tile_f32(
       const int jrun = min(jend, j + 64);
       float accf = 0.f;
       for (; j + 1 < jrun; j += 2) {
-        step(pr.off_c + j, ax, ay, az, al, acol, bx, by, bz, bl, bcol, accf, corr);
-        step(pr.off_c + j + 1, bx, by, bz, bl, bcol, ax, ay, az, al, acol, accf, corr);
+        step(pr.off_c + j, an, al, acol, bn, bl, bcol, accf, corr);
+        step(pr.off_c + j + 1, bn, bl, bcol, an, al, acol, accf, corr);
       }
       if (j < jrun) {
-        step(pr.off_c + j, ax, ay, az, al, acol, bx, by, bz, bl, bcol, accf, corr);
+        step(pr.off_c + j, an, al, acol, bn, bl, bcol, accf, corr);
 #pragma unroll
         for (int k = 0; k <= R; ++k) {
-          ax[k] = bx[k]; ay[k] = by[k]; az[k] = bz[k]; al[k] = bl[k];
+          an[k] = bn[k]; al[k] = bl[k];
         }
 #pragma unroll
         for (int k = 0; k < R; ++k) acol[k] = bcol[k];
@@ -364,7 +390,8 @@ __global__ __launch_bounds__(256) void convert_vertices(const double4 *__restric
     bad = (lx != 0.f) || (ly != 0.f) || (lz != 0.f);
     if (v + 1 < n) {
       const double4 q = d[v + 1];
-      ed[v] = make_float4((float)(q.x - p.x), (float)(q.y - p.y), (float)(q.z - p.z), 0.f);
+      const double ex = q.x - p.x, ey = q.y - p.y, ez = q.z - p.z;
+      ed[v] = make_float4((float)ex, (float)ey, (float)ez, (float)(-0.5 * (ex * ex + ey * ey + ez * ez)));
     } else {
       ed[v] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
