@@ -52,6 +52,7 @@ struct Mom64 {      // FP64 moments during construction
   double Q[18];
 };
 
+
 namespace {
 
 constexpr float kInv2PiF = 0.15915494309189533577f;
@@ -278,6 +279,79 @@ __global__ __launch_bounds__(128) void bh_moments(const BhRange *__restrict__ ra
     m64[id] = M;
   }
   store_node(M, ch, rr, geo, mom, id);
+}
+
+// Moments of the nodes of the bottom levels, where every node has at most
+// `direct` segments, in one launch over all of them: the box and centre in
+// FP64 as above, each segment's tangent and centred midpoint rounded to FP32
+// from FP64, and the closed forms accumulated in FP32 (at most `direct` terms,
+// the precision of the FP32 records they end in).  Registers stay low enough
+// for full occupancy; the general kernel above keeps FP64 for the nodes whose
+// sums run over many segments.
+__global__ __launch_bounds__(256) void bh_moments_direct(const BhRange *__restrict__ range, int32_t base,
+                                                         int32_t n, const int2 *__restrict__ child,
+                                                         const uint32_t *__restrict__ order,
+                                                         const double4 *__restrict__ pts, BhGeo *__restrict__ geo,
+                                                         BhMom *__restrict__ mom) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int id = base + k;
+  const BhRange rr = range[id];
+  const int2 ch = child[id];
+  double box[6];
+  leaf_box(rr, order, pts, box);
+  double ctr[3], dsq = 0;
+  for (int d = 0; d < 3; ++d) {
+    ctr[d] = 0.5 * (box[3 + d] + box[d]);
+    const double e = box[3 + d] - box[d];
+    dsq += e * e;
+  }
+  BhMom m;
+  for (int t = 0; t < 3; ++t) m.c[t] = 0.f;
+  for (int t = 0; t < 9; ++t) m.D[t] = 0.f;
+  for (int t = 0; t < 18; ++t) m.Q[t] = 0.f;
+  for (int s = rr.lo; s < rr.hi; ++s) {
+    const double4 a = pts[order[s]], b = pts[order[s] + 1];
+    const double pa[3] = {a.x, a.y, a.z}, pb[3] = {b.x, b.y, b.z};
+    float t[3], w[3];
+    for (int d = 0; d < 3; ++d) {
+      t[d] = (float)(pb[d] - pa[d]);
+      w[d] = (float)(0.5 * (pa[d] + pb[d]) - ctr[d]);
+    }
+    for (int i = 0; i < 3; ++i) {
+      m.c[i] += t[i];
+      for (int j = 0; j < 3; ++j) m.D[3 * i + j] = fmaf(t[i], w[j], m.D[3 * i + j]);
+    }
+    for (int kk = 0; kk < 3; ++kk)
+      for (int i = 0; i < 3; ++i)
+        for (int j = i; j < 3; ++j)
+          m.Q[6 * kk + qidx(i, j)] =
+              fmaf(t[kk], fmaf((1.0f / 12.0f) * t[i], t[j], w[i] * w[j]), m.Q[6 * kk + qidx(i, j)]);
+  }
+  m.pad[0] = m.pad[1] = 0.f;
+  BhGeo gr;
+  const float hx = (float)ctr[0], hy = (float)ctr[1], hz = (float)ctr[2];
+  gr.cx = hx; gr.cy = hy; gr.cz = hz;
+  gr.lx = (float)(ctr[0] - hx); gr.ly = (float)(ctr[1] - hy); gr.lz = (float)(ctr[2] - hz);
+  gr.rsq = (float)(0.25 * dsq);
+  gr.radius = sqrtf(gr.rsq);
+  gr.c0 = ch.x;
+  gr.c1 = ch.y;
+  gr.seg0 = rr.lo;
+  gr.nseg = rr.hi - rr.lo;
+  float dn = 0.f, qn = 0.f;
+  for (int t = 0; t < 9; ++t) dn = fmaf(m.D[t], m.D[t], dn);
+  for (int kk = 0; kk < 3; ++kk)
+    for (int t = 0; t < 6; ++t) {
+      const float v = m.Q[6 * kk + t];
+      qn = fmaf((t == 1 || t == 2 || t == 4) ? 2.f * v : v, v, qn);
+    }
+  gr.cn = sqrtf(fmaf(m.c[0], m.c[0], fmaf(m.c[1], m.c[1], m.c[2] * m.c[2])));
+  gr.dn = sqrtf(dn) * 0.57735026918962576451f;
+  gr.qn = (1.0f / 3.0f) * sqrtf(qn);
+  gr.pad = 0.f;
+  geo[id] = gr;
+  mom[id] = m;
 }
 
 // ---------------------------------------------------------------- far field
@@ -851,13 +925,17 @@ void build_forest(flk_ctx *ctx, const std::vector<int32_t> &curves_in, int leaf,
   // segments splits into floor(m/2) and m - floor(m/2), so every level of a
   // curve's tree holds at most two distinct range sizes (m and m + 1).  With
   // the counts known, the levels are enqueued without reading anything back.
+  const int32_t direct = std::max(leaf, kDirectSegments);
   std::vector<int64_t> level_split;
+  std::vector<int64_t> level_max;  // largest range of each level
   {
     std::vector<std::pair<int64_t, int64_t>> sizes;  // (range size, count) of the current level
     std::vector<std::pair<int64_t, int64_t>> next;
     for (int32_t k = 0; k < nc; ++k) sizes.emplace_back(seg_off[k + 1] - seg_off[k], 1);
     for (;;) {
-      int64_t split = 0;
+      int64_t split = 0, mx = 0;
+      for (const auto &sc : sizes) mx = std::max(mx, sc.first);
+      level_max.push_back(mx);
       next.clear();
       for (const auto &sc : sizes) {
         if (sc.first <= leaf) continue;
@@ -908,14 +986,38 @@ void build_forest(flk_ctx *ctx, const std::vector<int32_t> &curves_in, int leaf,
     total += 2 * nsplit;
   }
   F.nnodes = total;
+  // FP64 records exist only for nodes above `direct` segments: node ids are
+  // assigned level by level, so they all fall below the first level whose
+  // ranges are all that small
+  int64_t m64_nodes = total;
+  for (size_t lev = 0; lev < level_base.size() && lev < level_max.size(); ++lev)
+    if (level_max[lev] <= direct) {
+      m64_nodes = level_base[lev];
+      break;
+    }
   DevBuf &d_m64 = B.m64;
-  d_m64.reserve(sizeof(Mom64) * total);
+  d_m64.reserve(sizeof(Mom64) * std::max<int64_t>(m64_nodes, 1));
+  // node records of every level, sized before any moment kernel writes them
   F.geo.reserve(sizeof(BhGeo) * total);
   F.mom.reserve(sizeof(BhMom) * total);
-  for (int lev = (int)level_base.size() - 1; lev >= 0; --lev) {
+  // the bottom levels (every range within `direct`) in one launch
+  int lev_top = (int)level_base.size();
+  for (int lev = 0; lev < (int)level_base.size() && lev < (int)level_max.size(); ++lev)
+    if (level_max[lev] <= direct) {
+      lev_top = lev;
+      break;
+    }
+  if (lev_top < (int)level_base.size()) {
+    const int32_t b0 = level_base[lev_top], nd = total - b0;
+    hipLaunchKernelGGL(bh_moments_direct, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, s,
+                       d_range.as<BhRange>(), b0, nd, d_child.as<int2>(), F.order.as<uint32_t>(),
+                       cv.dpts.as<double4>(), F.geo.as<BhGeo>(), F.mom.as<BhMom>());
+    FLK_HIP(hipGetLastError());
+  }
+  for (int lev = lev_top - 1; lev >= 0; --lev) {
     const int32_t n = level_n[lev];
     hipLaunchKernelGGL(bh_moments, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, d_range.as<BhRange>(),
-                       n, level_base[lev], std::max(leaf, kDirectSegments), d_child.as<int2>(), F.order.as<uint32_t>(),
+                       n, level_base[lev], direct, d_child.as<int2>(), F.order.as<uint32_t>(),
                        cv.dpts.as<double4>(), d_m64.as<Mom64>(), F.geo.as<BhGeo>(), F.mom.as<BhMom>());
     FLK_HIP(hipGetLastError());
   }
@@ -1218,10 +1320,11 @@ void bh_batch(flk_ctx *ctx, const int32_t *pairs, const std::vector<int64_t> &bh
   }
 }
 
-// Device bytes the trees and their build records take per segment (two
-// 192-byte node records, one 288-byte FP64 record per internal node, ranges,
-// children, sort keys and scan flags), with headroom.
-constexpr double kBhBytesPerSegment = 800.0;
+// Device bytes the trees and their build records take per segment: two
+// 192-byte node records (384), 288-byte FP64 records for the nodes above 8
+// segments (about one node in eight: ~72), ranges and children (32), sort keys,
+// ids and order (24), scan flags (8) and sort scratch, with headroom.
+constexpr double kBhBytesPerSegment = 600.0;
 
 }  // namespace
 
