@@ -218,9 +218,9 @@ struct flk_ctx {
   flk::MlpState mlp;
   flk::BhScratch bh;
   flk::HostStage stage;
-  // default shapes: measured best on the 1,600-loop chain mail (R = 4 with
-  // every block resident: 985 ms against 1,016 ms for R = 8)
-  flk_tune tune{4, 0, 8, 4, 0, {0, 0, 0}};
+  // default shapes: measured best on the 1,600-loop chain mail (R = 2 with
+  // every block resident: 880 ms against 931 ms for R = 4, 1,068 ms for R = 8)
+  flk_tune tune{2, 0, 8, 4, 0, {0, 0, 0}};
   flk::DsGraph ds_graph[2];      // FP32, FP64 batches
   flk::DiscResult disc;          // last flk_discretize result
   uint64_t curves_version = 0;   // bumped by every flk_set_curves
