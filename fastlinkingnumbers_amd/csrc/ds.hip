@@ -158,7 +158,13 @@ __global__ __launch_bounds__(256) void ds_tile_f32(
       }
     };
     const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    const int64_t vlast = pr.off_c + pr.n_c;  // closing vertex of the column curve
+    // column vertex indices are clamped to the closing vertex (index n_c of
+    // the curve) in 32 bits, which stays on the scalar unit (there is no
+    // 64-bit scalar compare)
+    const float4 *__restrict__ chi = hi + pr.off_c;
+    const float4 *__restrict__ clo = lo + pr.off_c;
+    const float4 *__restrict__ ced = ed + pr.off_c;
+    const int ncl = pr.n_c;
     float an[R + 1], al[R + 1], acol[R];
     float bn[R + 1], bl[R + 1], bcol[R];
     {
@@ -177,14 +183,14 @@ __global__ __launch_bounds__(256) void ds_tile_f32(
 
     // One column step: the new column (j+1), then the R segment pairs (k, j)
     // of the quadrilaterals a=G(k,j) b=G(k+1,j) c=G(k+1,j+1) d=G(k,j+1).
-    auto step = [&](int64_t v, const float (&gn)[R + 1], const float (&gl)[R + 1], const float (&gcol)[R],
+    auto step = [&](int j, const float (&gn)[R + 1], const float (&gl)[R + 1], const float (&gcol)[R],
                     float (&nn)[R + 1], float (&nl)[R + 1], float (&ncol)[R], float &accf, int &corr) {
       const float4 q = nq, ql = nql, e2 = ne;
       {
-        const int64_t w = min(v + 2, vlast);
-        nq = hi[w];
-        if constexpr (PRECISE) nql = lo[w];
-        ne = ed[min(v + 1, vlast)];
+        const int w = min(j + 2, ncl);
+        nq = chi[w];
+        if constexpr (PRECISE) nql = clo[w];
+        ne = ced[min(j + 1, ncl)];
       }
       float nx[R + 1], ny[R + 1], nz[R + 1];
       column(q, ql, nx, ny, nz, nn, nl);
@@ -239,7 +245,8 @@ __global__ __launch_bounds__(256) void ds_tile_f32(
         zi[c] = fmaf(xp[2 * c], yp[2 * c + 1], yp[2 * c] * xp[2 * c + 1]);
         small = small && (zr[c] > 1e-30f);
       }
-      if (__all(small)) {
+      // every lane on the short path: a scalar test of the lane mask
+      if (__builtin_amdgcn_ballot_w64(!small) == 0) {
         // atan(t) = t P(t^2) on |t| <= 16/63: minimax coefficients fitted for
         // this engine (relative error 2e-10 in exact arithmetic, no bias)
 #pragma unroll
@@ -269,11 +276,11 @@ __global__ __launch_bounds__(256) void ds_tile_f32(
       const int jrun = min(jend, j + 64);
       float accf = 0.f;
       for (; j + 1 < jrun; j += 2) {
-        step(pr.off_c + j, an, al, acol, bn, bl, bcol, accf, corr);
-        step(pr.off_c + j + 1, bn, bl, bcol, an, al, acol, accf, corr);
+        step(j, an, al, acol, bn, bl, bcol, accf, corr);
+        step(j + 1, bn, bl, bcol, an, al, acol, accf, corr);
       }
       if (j < jrun) {
-        step(pr.off_c + j, an, al, acol, bn, bl, bcol, accf, corr);
+        step(j, an, al, acol, bn, bl, bcol, accf, corr);
 #pragma unroll
         for (int k = 0; k <= R; ++k) {
           an[k] = bn[k]; al[k] = bl[k];
