@@ -82,3 +82,25 @@ def test_f32_and_f64_inputs_give_identical_results(engine):
 def test_tune_rejects_bad_thread_count(engine, restore_tune):
     with pytest.raises(flk.FlkError, match="ingest_threads"):
         engine.set_tune(ingest_threads=-1)
+
+
+def test_f32_upload_edge_cases(engine):
+    """Empty curves, a single vertex, 2-3 vertex curves and an all-empty model
+    through the FP32 entry point give the same boxes and linking numbers as
+    the same values through the FP64 one."""
+    t = np.linspace(0, 2 * np.pi, 64, endpoint=False)
+    circle = np.stack([np.cos(t), np.sin(t), 0 * t], 1)
+    tiny = [np.zeros((0, 3)), np.array([[0.0, 0.0, 0.0]]), np.array([[0.0, 0, -1], [0.0, 0, 1]]),
+            np.array([[0.5, 0, -1], [0.5, 0, 1], [2.0, 0, 0]])]
+    xyz, offs = flk.pack_curves([circle] + tiny)
+    xyz = xyz.astype(np.float32).astype(np.float64)
+    pairs = np.array([[0, k] for k in range(1, 5)], np.int32)
+    res = {}
+    for dt in (np.float64, np.float32):
+        engine.set_curves_packed(xyz.astype(dt), offs)
+        res[dt] = (engine.curve_boxes(), engine.linking_numbers(pairs, method="directsum").lk)
+    assert np.array_equal(res[np.float64][0], res[np.float32][0], equal_nan=True)
+    assert np.array_equal(res[np.float64][1], res[np.float32][1])
+    for dt in (np.float64, np.float32):
+        engine.set_curves_packed(np.zeros((0, 3), dt), np.zeros(4, np.int64))
+        assert engine.ncurves == 3 and engine.potential_links().shape == (0, 2)
