@@ -82,6 +82,18 @@ class FlkStats(C.Structure):
 
 # Every function declared in include/flk.h: (name, restype, argtypes).
 _P = C.POINTER
+class FlkBccInfo(C.Structure):
+    """Result of flk_load_bcc (include/flk.h flk_bcc_info)."""
+    _fields_ = [
+        ("ncurves", C.c_int32),
+        ("segment_type", C.c_int32),
+        ("points", C.c_int64),
+        ("offsets", C.POINTER(C.c_int64)),
+        ("looped", C.POINTER(C.c_uint8)),
+        ("reserved", C.c_int64 * 4),
+    ]
+
+
 class FlkTune(C.Structure):
     """Kernel launch shapes (include/flk.h flk_tune)."""
     _fields_ = [
@@ -123,6 +135,7 @@ FLK_EXPORTS = [
     ("flk_device_name", C.c_int32, [C.c_void_p, C.c_char_p, C.c_int32]),
     ("flk_set_curves", C.c_int32, [C.c_void_p, _P(C.c_double), _P(C.c_int64), C.c_int32]),
     ("flk_set_curves_f32", C.c_int32, [C.c_void_p, _P(C.c_float), _P(C.c_int64), C.c_int32]),
+    ("flk_load_bcc", C.c_int32, [C.c_void_p, C.c_char_p, _P(FlkBccInfo)]),
     ("flk_curve_boxes", C.c_int32, [C.c_void_p, _P(C.c_double)]),
     ("flk_potential_links", C.c_int32,
      [C.c_void_p, _P(C.c_double), C.c_int32, _P(_P(C.c_int32)), _P(C.c_int64)]),
@@ -270,6 +283,38 @@ class Engine:
                         "flk_set_curves")
         self.ncurves = n
         self.sizes = np.diff(offs)
+
+    def load_bcc(self, path: str) -> dict:
+        """Reads a BCC file straight into the engine's curve set (control
+        points); returns the segment type, the offsets and the loop flags."""
+        info = FlkBccInfo()
+        self._check(self.lib.flk_load_bcc(self.h, os.fsencode(path), C.byref(info)), "flk_load_bcc")
+        n = info.ncurves
+        offs = np.ctypeslib.as_array(info.offsets, shape=(n + 1,)).copy() if n else np.zeros(1, np.int64)
+        looped = np.ctypeslib.as_array(info.looped, shape=(n,)).copy().astype(bool) if n else np.zeros(0, bool)
+        self.ncurves = n
+        self.sizes = np.diff(offs)
+        return {"segment_type": info.segment_type, "points": info.points, "offsets": offs, "looped": looped}
+
+    def discretize_current(self, types, looped, pairs):
+        """Device discretization of the engine's current curve set (control
+        points, e.g. from load_bcc); the polylines replace it."""
+        n = self.ncurves
+        types = np.broadcast_to(np.asarray(types, dtype=np.int32), (n,)).copy()
+        looped = np.broadcast_to(np.asarray(looped, dtype=np.uint8), (n,)).copy()
+        offs = np.zeros(n + 1, np.int64)
+        offs[1:] = np.cumsum(self.sizes)
+        pairs = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1, 2))
+        ooff = C.POINTER(C.c_int64)()
+        info = FlkDiscInfo()
+        rc = self.lib.flk_discretize(self.h, None, _ptr(offs, C.c_int64), _ptr(types, C.c_int32),
+                                     _ptr(looped, C.c_uint8), n, _ptr(pairs, C.c_int32), len(pairs),
+                                     None, C.byref(ooff), C.byref(info))
+        self._check(rc, "flk_discretize")
+        off = np.ctypeslib.as_array(ooff, shape=(n + 1,)).copy()
+        self.sizes = np.diff(off)
+        return {"rounds": info.rounds, "overlaps": list(info.overlaps[: info.rounds]), "segments": info.segments,
+                "pieces": info.pieces, "vertices": info.vertices}
 
     def curve_boxes(self) -> np.ndarray:
         out = np.zeros((self.ncurves, 6), dtype=np.float64)

@@ -369,7 +369,19 @@ int32_t flk_discretize(flk_ctx *ctx, const double *ctrl_xyz, const int64_t *offs
   try {
     FLK_HIP(hipSetDevice(ctx->device));
     if (ncurves > 0 && offsets[0] != 0) throw Error(FLK_ERR_ARG, "offsets[0] must be 0");
-    if (ncurves > 0 && offsets[ncurves] > 0 && !ctrl_xyz) throw Error(FLK_ERR_ARG, "ctrl_xyz is NULL");
+    std::vector<double> current;
+    if (!ctrl_xyz && ncurves > 0 && offsets[ncurves] > 0) {
+      // the control points are the current curve set's input array; the
+      // reference's guard and extent loops read them on the host
+      const flk::Curves &cv = ctx->curves;
+      if (cv.ncurves != ncurves || !std::equal(offsets, offsets + ncurves + 1, cv.in_off.begin()))
+        throw Error(FLK_ERR_ARG, "offsets do not match the current curve set");
+      current.resize(3 * (size_t)offsets[ncurves]);
+      FLK_HIP(hipMemcpyAsync(current.data(), cv.raw.ptr, sizeof(double) * current.size(), hipMemcpyDeviceToHost,
+                             ctx->stream));
+      FLK_HIP(hipStreamSynchronize(ctx->stream));
+      ctrl_xyz = current.data();
+    }
     flk::discretize_run(ctx, ctrl_xyz, offsets, segment_types, looped, ncurves, pairs, npairs, ctx->disc,
                         out_xyz != nullptr);
   } catch (const flk::Error &e) {
@@ -388,6 +400,23 @@ int32_t flk_discretize(flk_ctx *ctx, const double *ctrl_xyz, const int64_t *offs
   if (out_xyz) *out_xyz = ctx->disc.xyz.data();
   if (out_offsets) *out_offsets = ctx->disc.out_offsets.data();
   return FLK_OK;
+}
+
+int32_t flk_load_bcc(flk_ctx *ctx, const char *path, flk_bcc_info *info) {
+  if (!ctx || !path) return FLK_ERR_ARG;
+  FLK_GUARD(ctx, {
+    FLK_HIP(hipSetDevice(ctx->device));
+    flk::load_bcc(ctx, path, ctx->bcc);
+    if (info) {
+      std::memset(info, 0, sizeof *info);
+      info->ncurves = ctx->curves.ncurves;
+      info->segment_type = ctx->bcc.type;
+      info->points = ctx->bcc.points;
+      info->offsets = ctx->bcc.offsets.data();
+      info->looped = ctx->bcc.looped.data();
+    }
+    return FLK_OK;
+  })
 }
 
 int32_t flk_get_tune(const flk_ctx *ctx, flk_tune *out) {

@@ -186,6 +186,15 @@ struct DsGraph {
 };
 
 // Result of a device discretization (context-owned).
+// A BCC file read into the context's curve set: segment type, per-curve
+// offsets and loop flags.
+struct BccLoad {
+  int32_t type = 0;
+  int64_t points = 0;
+  std::vector<int64_t> offsets;
+  std::vector<uint8_t> looped;
+};
+
 struct DiscResult {
   int64_t segments = 0, retired = 0;
   int32_t rounds = 0;
@@ -223,6 +232,8 @@ struct flk_ctx {
   flk_tune tune{2, 0, 8, 4, 0, {0, 0, 0}};
   flk::DsGraph ds_graph[2];      // FP32, FP64 batches
   flk::DiscResult disc;          // last flk_discretize result
+  flk::BccLoad bcc;              // last flk_load_bcc result
+  flk::DevBuf bcc_land;          // FP32 landing area of flk_load_bcc
   uint64_t curves_version = 0;   // bumped by every flk_set_curves
   // scratch
   flk::DevBuf d_pairs, d_tasks, d_cost, d_task_theta, d_task_corr, d_pair_task0,
@@ -274,6 +285,11 @@ void curves_clear(flk_ctx *ctx);
 // bytes sent.
 int64_t ingest_points(flk_ctx *ctx, const void *src, bool src_f32, int64_t n, double *dst, float *land,
                       size_t land_bytes);
+// dst[i] = (double)land[i], i < nval, on the context's stream.
+void widen_launch(flk_ctx *ctx, const float *land, int64_t nval, double *dst);
+
+// Reads a BCC file into the context's curve set (see ingest.hip).
+void load_bcc(flk_ctx *ctx, const char *path, BccLoad &out);
 void convert_vertices_launch(hipStream_t s, const double4 *d, int64_t n, float4 *hi, float4 *lo,
                              float4 *ed, int32_t *inexact);
 

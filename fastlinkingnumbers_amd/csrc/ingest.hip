@@ -14,7 +14,10 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
+#include <cstdio>
 #include <cstring>
+#include <string>
 #include <thread>
 
 #include "flk_internal.h"
@@ -69,6 +72,13 @@ void HostStage::release() {
   buf.clear();
   done.clear();
   pending.clear();
+}
+
+void widen_launch(flk_ctx *ctx, const float *land, int64_t nval, double *dst) {
+  if (nval <= 0) return;
+  const int64_t blocks = std::min<int64_t>((nval + 255) / 256, 64 * (int64_t)std::max(ctx->num_cus, 1));
+  hipLaunchKernelGGL(widen_f32, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, land, nval, dst);
+  FLK_HIP(hipGetLastError());
 }
 
 int ingest_workers(const flk_ctx *ctx) {
@@ -147,14 +157,112 @@ int64_t ingest_points(flk_ctx *ctx, const void *src, bool src_f32, int64_t n, do
       for (auto &p : st.pending) p = 0;
       continue;
     }
-    if (wire_f32) {
-      const int64_t blocks = std::min<int64_t>((nval + 255) / 256, 64 * (int64_t)std::max(ctx->num_cus, 1));
-      hipLaunchKernelGGL(widen_f32, dim3((unsigned)blocks), dim3(256), 0, s, land, nval, dst);
-      FLK_HIP(hipGetLastError());
-    }
+    if (wire_f32) widen_launch(ctx, land, nval, dst);
     return (int64_t)esz * nval;
   }
   return 0;  // not reached
+}
+
+// BCC file (reference src/model.cpp:266-364): a 64-byte header, then per
+// curve an int32 control-point count (negative: closed loop) and that many
+// FP32 xyz triples.  The coordinates are read straight into the pinned
+// staging buffers, the counts stripped on the way, and each full buffer is
+// sent while the next one is read; once the counts are known the curve set is
+// laid out and the FP32 array widened on the device.
+void load_bcc(flk_ctx *ctx, const char *path, BccLoad &out) {
+  out = BccLoad{};
+  std::FILE *f = std::fopen(path, "rb");
+  if (!f) throw Error(FLK_ERR_ARG, std::string("Error opening input file: ") + path);
+  struct Closer {
+    std::FILE *f;
+    ~Closer() { std::fclose(f); }
+  } closer{f};
+  auto fail = [&](const char *why) { throw Error(FLK_ERR_ARG, std::string(why) + ": " + path); };
+#pragma pack(push, 1)
+  struct Header {
+    char sign[3];
+    unsigned char bytes;
+    char type[2];
+    char dims;
+    char up;
+    uint64_t ncurves;
+    uint64_t npoints;
+    char info[40];
+  } h;
+#pragma pack(pop)
+  static_assert(sizeof(Header) == 64, "BCC header is 64 bytes");
+  if (std::fread(&h, sizeof h, 1, f) != 1) fail("Incomplete header");
+  if (h.sign[0] != 'B' || h.sign[1] != 'C' || h.sign[2] != 'C') fail("Invalid header");
+  if (h.bytes != 0x44) fail("Invalid bytecount. We only support 32-bit integers and floats.");
+  if (h.dims != 3) fail("Invalid dimensions. We only support data in 3D Cartesian space.");
+  if (h.type[0] == 'C' && h.type[1] == '0') out.type = FLK_SEG_CATMULLROM;
+  else if (h.type[0] == 'B' && h.type[1] == 'S') out.type = FLK_SEG_BSPLINE;
+  else if (h.type[0] == 'B' && h.type[1] == 'Z') out.type = FLK_SEG_BEZIER;
+  else if (h.type[0] == 'P' && h.type[1] == 'L') out.type = FLK_SEG_POLYLINE;
+  else fail("Invalid valid curve segment type. We only support C0, BS, BZ, and PL.");
+  if (h.ncurves > (uint64_t)INT32_MAX - 1) fail("Too many curves");
+  const int32_t nc = (int32_t)h.ncurves;
+  // the coordinates cannot exceed what the file holds after the counts
+  if (std::fseek(f, 0, SEEK_END) != 0) fail("Cannot read file size");
+  const long long fsize = std::ftell(f);
+  if (std::fseek(f, (long)sizeof h, SEEK_SET) != 0 || fsize < 0) fail("Cannot read file size");
+  const long long coord_bytes = fsize - (long long)sizeof h - 4LL * nc;
+  if (coord_bytes < 0) fail("Cannot read curve control point count");
+  hipStream_t s = ctx->stream;
+  ctx->bcc_land.reserve(std::max<size_t>((size_t)coord_bytes, 16));
+  char *land = static_cast<char *>(ctx->bcc_land.ptr);
+  HostStage &st = ctx->stage;
+  st.ensure(1);
+  int b = 0;
+  size_t fill = 0;
+  int64_t sent = 0;  // bytes handed to DMA so far
+  auto flush = [&]() {
+    if (fill == 0) return;
+    FLK_HIP(hipMemcpyAsync(land + sent, st.buf[b], fill, hipMemcpyHostToDevice, s));
+    FLK_HIP(hipEventRecord(st.done[b], s));
+    st.pending[b] = 1;
+    sent += (int64_t)fill;
+    fill = 0;
+    b ^= 1;
+    if (st.pending[b]) {  // the other buffer's DMA must end before it is refilled
+      FLK_HIP(hipEventSynchronize(st.done[b]));
+      st.pending[b] = 0;
+    }
+  };
+  if (st.pending[b]) {
+    FLK_HIP(hipEventSynchronize(st.done[b]));
+    st.pending[b] = 0;
+  }
+  out.offsets.assign(1, 0);
+  out.looped.assign(nc, 0);
+  for (int32_t c = 0; c < nc; ++c) {
+    int32_t cnt;
+    if (std::fread(&cnt, sizeof cnt, 1, f) != 1) fail("Cannot read curve control point count");
+    out.looped[c] = cnt < 0 ? 1 : 0;
+    const int64_t n = cnt < 0 ? -(int64_t)cnt : (int64_t)cnt;
+    if (12 * n > coord_bytes - (sent + (int64_t)fill)) fail("Invalid number of control points read");
+    size_t left = (size_t)(12 * n);
+    while (left > 0) {
+      const size_t take = std::min(left, HostStage::kChunkBytes - fill);
+      if (std::fread(static_cast<char *>(st.buf[b]) + fill, 1, take, f) != take)
+        fail("Invalid number of control points read");
+      fill += take;
+      left -= take;
+      if (fill == HostStage::kChunkBytes) flush();
+    }
+    out.offsets.push_back(out.offsets.back() + n);
+  }
+  flush();
+  out.points = out.offsets.back();
+  // the curve set: layout from the counts, then the FP32 array widened
+  try {
+    curves_layout(ctx, out.offsets.data(), nc);
+    widen_launch(ctx, reinterpret_cast<const float *>(land), 3 * out.points, ctx->curves.raw.as<double>());
+    curves_finish(ctx);
+  } catch (...) {
+    curves_clear(ctx);
+    throw;
+  }
 }
 
 }  // namespace flk

@@ -170,7 +170,8 @@ typedef struct flk_disc_info {
  * curve (pairs: i < j, as from flk_potential_links on the curves' spline
  * boxes) until no overlap is left, then returns each curve as a polyline of
  * its pieces' start points (an open curve also gets its last end point).
- * ctrl_xyz: control points, AoS, curve c = rows offsets[c]..offsets[c+1];
+ * ctrl_xyz: control points, AoS, curve c = rows offsets[c]..offsets[c+1]
+ * (NULL: the context's current curve set, whose offsets must be given);
  * segment_types: FLK_SEG_*; looped: 0/1 per curve.  On success the polylines
  * become the context's curve set, as after flk_set_curves on them (they stay
  * on the device: no copy back and up again), and *out_xyz / *out_offsets
@@ -183,6 +184,30 @@ int32_t flk_discretize(flk_ctx *ctx, const double *ctrl_xyz, const int64_t *offs
                        const int32_t *segment_types, const uint8_t *looped, int32_t ncurves,
                        const int32_t *pairs, int64_t npairs, const double **out_xyz,
                        const int64_t **out_offsets, flk_disc_info *info);
+
+/* Result of flk_load_bcc.  offsets and looped are context-owned, valid until
+ * the next flk_load_bcc. */
+typedef struct flk_bcc_info {
+  int32_t ncurves;
+  int32_t segment_type;     /* FLK_SEG_* of every curve of the file */
+  int64_t points;           /* control points over all curves */
+  const int64_t *offsets;   /* ncurves + 1: curve c = points offsets[c]..offsets[c+1] */
+  const uint8_t *looped;    /* ncurves: 1 for a closed loop (negative count in the file) */
+  int64_t reserved[4];
+} flk_bcc_info;
+
+/* Reads a BCC file straight into the context's curve set, as
+ * flk_set_curves_f32 on its control points would: the reference's
+ * Model::ImportFromFile for BCC (src/model.cpp:280-364, same header checks and
+ * messages) without the FP64 Curve objects.  The file is read in chunks into
+ * pinned buffers whose DMA overlaps the reading of the next chunk.  For a
+ * polyline file the curves are ready for flk_potential_links and
+ * flk_linking_numbers; for a spline file they are the control polygons, which
+ * flk_discretize(ctx, NULL, info->offsets, ...) discretizes in place.  A
+ * file that fails to read (status FLK_ERR_ARG and the reference's message:
+ * header, counts, truncated coordinates) leaves the previous curve set in
+ * place; a device failure after the file was read leaves an empty one. */
+int32_t flk_load_bcc(flk_ctx *ctx, const char *path, flk_bcc_info *info);
 
 /* Launch shapes of the kernels.  No reference counterpart (the reference is
  * CPU-only): flk_autotune picks them by timing candidates on the caller's own

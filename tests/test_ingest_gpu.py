@@ -2,7 +2,11 @@
 holds exactly travels as FP32, any other FP64 input as FP64, FP32 input as
 is.  Every route must leave the same FP64 vertices on the device -- read back
 through a discretization without potential links, which returns each closed
-polyline's vertices unchanged -- and give the same linking numbers."""
+polyline's vertices unchanged -- and give the same linking numbers.  BCC files
+stream straight into the curve set (flk_load_bcc) with the same result as the
+oracle's import of the file."""
+import os
+
 import numpy as np
 import pytest
 
@@ -104,3 +108,65 @@ def test_f32_upload_edge_cases(engine):
     for dt in (np.float64, np.float32):
         engine.set_curves_packed(np.zeros((0, 3), dt), np.zeros(4, np.int64))
         assert engine.ncurves == 3 and engine.potential_links().shape == (0, 2)
+
+
+def _boxes(curves):
+    return np.array([np.r_[c.min(0), c.max(0)] if len(c) else np.r_[[np.inf] * 3, [-np.inf] * 3]
+                     for c in curves])
+
+
+@pytest.mark.parametrize("name", ["glove", "knittubeinit", "knittubebroken"])
+def test_load_bcc_streams_fixture_and_discretizes_in_place(engine, name):
+    """flk_load_bcc reads the reference's BCC fixtures into the curve set: the
+    same segment type, loop flags and control points as the oracle's import of
+    the file; discretizing that set in place gives the same polylines as the
+    host-array route, and the certificate of the reference's fixture."""
+    from conftest import GOLDEN
+    path = os.path.join(GOLDEN, "data", name + ".bcc")
+    m = O.Model.load_bcc(path)
+    info = engine.load_bcc(path)
+    n = len(m)
+    assert info["segment_type"] == m.info(0)[0] and info["points"] == sum(len(c) for c in m.curves())
+    assert list(info["looped"]) == [m.info(c)[1] for c in range(n)]
+    assert np.array_equal(engine.curve_boxes(), _boxes(m.curves()))
+    pairs = m.pls()  # potential links on the spline boxes
+    types = [m.info(c)[0] for c in range(n)]
+    d = engine.discretize_current(types, info["looped"], pairs)
+    sizes = engine.sizes.copy()
+    ref, dref = engine.discretize(m.curves(), types, info["looped"], pairs)
+    assert d["overlaps"] == dref["overlaps"] and list(sizes) == [len(c) for c in ref]
+    got = engine.potential_links()
+    res = engine.linking_numbers(got, method="barneshut")
+    with open(os.path.join(GOLDEN, "reference_certificates", name + ".txt")) as f:
+        lines = f.read().split()
+    want = [tuple(int(v) for v in l.split(",")) for l in lines[1:]]
+    assert int(lines[0]) == n and flk.certificate_from_lk(got, res.lk) == want
+
+
+def test_load_bcc_polyline_and_errors(engine, tmp_path):
+    """A polyline BCC written by the oracle loads to the same curves and
+    linking numbers as the host-array route; unreadable, foreign and truncated
+    files fail with the reference's messages, a rejected header keeping the
+    previous curve set."""
+    curves = [c.astype(np.float32).astype(np.float64) for c in W.hopf_torus_circles(seed=7)]
+    path = str(tmp_path / "hopf.bcc")
+    O.Model.from_curves(curves).save_bcc(path)
+    info = engine.load_bcc(path)
+    assert info["segment_type"] == flk.SEG_POLYLINE and all(info["looped"])
+    pairs = np.array([[0, 1], [2, 3]], np.int32)
+    a = engine.linking_numbers(pairs, method="directsum").lk
+    boxes = engine.curve_boxes()
+    engine.set_curves(curves)
+    assert np.array_equal(boxes, engine.curve_boxes())
+    assert np.array_equal(a, engine.linking_numbers(pairs, method="directsum").lk)
+    with pytest.raises(flk.FlkError, match="Error opening input file"):
+        engine.load_bcc(str(tmp_path / "missing.bcc"))
+    from conftest import GOLDEN
+    with pytest.raises(flk.FlkError, match="Invalid header"):
+        engine.load_bcc(os.path.join(GOLDEN, "data", "blenderobj.obj"))
+    assert np.array_equal(boxes, engine.curve_boxes())  # previous set kept
+    raw = open(path, "rb").read()
+    cut = str(tmp_path / "cut.bcc")
+    open(cut, "wb").write(raw[: len(raw) - 100])
+    with pytest.raises(flk.FlkError, match="Invalid number of control points read"):
+        engine.load_bcc(cut)
