@@ -210,94 +210,15 @@ __device__ void store_node(const Mom64 &M, int2 ch, BhRange r, BhGeo *__restrict
   mom[id] = m;
 }
 
-// Bottom-up moments of one level (children already done) and the node's FP32
-// records.  A node of at most `direct` segments (leaf buckets and the small
-// subtrees above them, which are most of the nodes) sums its segments' closed
-// forms about its own centre and keeps no FP64 record; a larger node adds
-// such a child's segments about its own centre directly (the closed form
-// shifted exactly) and a larger child's FP64 record with the shift formulas
-// of the reference (compute_link.cpp:358-389, Eqs. 21-23).  Only nodes above
-// `direct` segments write FP64 records, so the bottom levels never round-trip
-// them through memory.  Children are read one at a time.
-__global__ __launch_bounds__(128) void bh_moments(const BhRange *__restrict__ range, int32_t n,
-                                                  int32_t base, int32_t direct,
-                                                  const int2 *__restrict__ child,
-                                                  const uint32_t *__restrict__ order,
-                                                  const double4 *__restrict__ pts,
-                                                  Mom64 *__restrict__ m64, BhGeo *__restrict__ geo,
-                                                  BhMom *__restrict__ mom) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  const int id = base + k;
-  const int2 ch = child[id];
-  const BhRange rr = range[id];
-  Mom64 M;
-  for (int t = 0; t < 3; ++t) M.c[t] = 0;
-  for (int t = 0; t < 9; ++t) M.D[t] = 0;
-  for (int t = 0; t < 18; ++t) M.Q[t] = 0;
-  if (ch.x < 0 || rr.hi - rr.lo <= direct) {
-    leaf_box(rr, order, pts, M.box);
-    double ctr[3];
-    for (int d = 0; d < 3; ++d) ctr[d] = 0.5 * (M.box[3 + d] + M.box[d]);
-    add_segments(rr, order, pts, ctr, M);
-  } else {
-    const BhRange rc[2] = {range[ch.x], range[ch.y]};
-    const int cid[2] = {ch.x, ch.y};
-    double cb[2][6];
-    for (int side = 0; side < 2; ++side) {
-      if (rc[side].hi - rc[side].lo <= direct) leaf_box(rc[side], order, pts, cb[side]);
-      else
-        for (int t = 0; t < 6; ++t) cb[side][t] = m64[cid[side]].box[t];
-    }
-    for (int d = 0; d < 3; ++d) {
-      M.box[d] = fmin(cb[0][d], cb[1][d]);
-      M.box[3 + d] = fmax(cb[0][3 + d], cb[1][3 + d]);
-    }
-    double ctr[3];
-    for (int d = 0; d < 3; ++d) ctr[d] = 0.5 * (M.box[3 + d] + M.box[d]);
-    for (int side = 0; side < 2; ++side) {
-      if (rc[side].hi - rc[side].lo <= direct) {
-        add_segments(rc[side], order, pts, ctr, M);
-        continue;
-      }
-      const Mom64 &C = m64[cid[side]];
-      double r[3];
-      for (int d = 0; d < 3; ++d) r[d] = 0.5 * (cb[side][3 + d] + cb[side][d]) - ctr[d];
-      double cc[3];
-      for (int i = 0; i < 3; ++i) cc[i] = C.c[i];
-      for (int kk = 0; kk < 3; ++kk) {
-        double dk[3];
-        for (int j = 0; j < 3; ++j) dk[j] = C.D[3 * kk + j];
-        for (int i = 0; i < 3; ++i)
-          for (int j = i; j < 3; ++j)
-            M.Q[6 * kk + qidx(i, j)] += C.Q[6 * kk + qidx(i, j)] + r[j] * dk[i] + r[i] * dk[j] +
-                                        cc[kk] * (r[i] * r[j]);
-        for (int j = 0; j < 3; ++j) M.D[3 * kk + j] += dk[j] + cc[kk] * r[j];
-        M.c[kk] += cc[kk];
-      }
-    }
-    m64[id] = M;
-  }
-  store_node(M, ch, rr, geo, mom, id);
-}
-
-// Moments of the nodes of the bottom levels, where every node has at most
-// `direct` segments, in one launch over all of them: the box and centre in
-// FP64 as above, each segment's tangent and centred midpoint rounded to FP32
-// from FP64, and the closed forms accumulated in FP32 (at most `direct` terms,
-// the precision of the FP32 records they end in).  Registers stay low enough
-// for full occupancy; the general kernel above keeps FP64 for the nodes whose
-// sums run over many segments.
-__global__ __launch_bounds__(256) void bh_moments_direct(const BhRange *__restrict__ range, int32_t base,
-                                                         int32_t n, const int2 *__restrict__ child,
-                                                         const uint32_t *__restrict__ order,
-                                                         const double4 *__restrict__ pts, BhGeo *__restrict__ geo,
-                                                         BhMom *__restrict__ mom) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  const int id = base + k;
-  const BhRange rr = range[id];
-  const int2 ch = child[id];
+// Moments and records of a node of at most `direct` segments, from its
+// segments: the box and centre in FP64, each segment's tangent and centred
+// midpoint rounded to FP32 from FP64, and the closed forms accumulated in FP32
+// (at most `direct` terms, the precision of the FP32 records they end in).
+// Both moment kernels use it, so a node's records depend only on its own
+// segments, never on which launch computed them.
+__device__ void direct_node(BhRange rr, int2 ch, const uint32_t *__restrict__ order,
+                            const double4 *__restrict__ pts, BhGeo *__restrict__ geo, BhMom *__restrict__ mom,
+                            int id) {
   double box[6];
   leaf_box(rr, order, pts, box);
   double ctr[3], dsq = 0;
@@ -352,6 +273,89 @@ __global__ __launch_bounds__(256) void bh_moments_direct(const BhRange *__restri
   gr.pad = 0.f;
   geo[id] = gr;
   mom[id] = m;
+}
+
+// The bottom levels, where every node has at most `direct` segments, in one
+// launch over all of them; registers stay low enough for full occupancy.
+__global__ __launch_bounds__(256) void bh_moments_direct(const BhRange *__restrict__ range, int32_t base,
+                                                         int32_t n, const int2 *__restrict__ child,
+                                                         const uint32_t *__restrict__ order,
+                                                         const double4 *__restrict__ pts, BhGeo *__restrict__ geo,
+                                                         BhMom *__restrict__ mom) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int id = base + k;
+  direct_node(range[id], child[id], order, pts, geo, mom, id);
+}
+
+// Bottom-up moments of one level (children already done) and the node's FP32
+// records.  A node of at most `direct` segments (leaf buckets and the small
+// subtrees above them, which are most of the nodes) is direct_node's and keeps
+// no FP64 record; a larger node adds
+// such a child's segments about its own centre directly (the closed form
+// shifted exactly) and a larger child's FP64 record with the shift formulas
+// of the reference (compute_link.cpp:358-389, Eqs. 21-23).  Only nodes above
+// `direct` segments write FP64 records, so the bottom levels never round-trip
+// them through memory.  Children are read one at a time.
+__global__ __launch_bounds__(128) void bh_moments(const BhRange *__restrict__ range, int32_t n,
+                                                  int32_t base, int32_t direct,
+                                                  const int2 *__restrict__ child,
+                                                  const uint32_t *__restrict__ order,
+                                                  const double4 *__restrict__ pts,
+                                                  Mom64 *__restrict__ m64, BhGeo *__restrict__ geo,
+                                                  BhMom *__restrict__ mom) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int id = base + k;
+  const int2 ch = child[id];
+  const BhRange rr = range[id];
+  if (ch.x < 0 || rr.hi - rr.lo <= direct) {
+    direct_node(rr, ch, order, pts, geo, mom, id);
+    return;
+  }
+  Mom64 M;
+  for (int t = 0; t < 3; ++t) M.c[t] = 0;
+  for (int t = 0; t < 9; ++t) M.D[t] = 0;
+  for (int t = 0; t < 18; ++t) M.Q[t] = 0;
+  {
+    const BhRange rc[2] = {range[ch.x], range[ch.y]};
+    const int cid[2] = {ch.x, ch.y};
+    double cb[2][6];
+    for (int side = 0; side < 2; ++side) {
+      if (rc[side].hi - rc[side].lo <= direct) leaf_box(rc[side], order, pts, cb[side]);
+      else
+        for (int t = 0; t < 6; ++t) cb[side][t] = m64[cid[side]].box[t];
+    }
+    for (int d = 0; d < 3; ++d) {
+      M.box[d] = fmin(cb[0][d], cb[1][d]);
+      M.box[3 + d] = fmax(cb[0][3 + d], cb[1][3 + d]);
+    }
+    double ctr[3];
+    for (int d = 0; d < 3; ++d) ctr[d] = 0.5 * (M.box[3 + d] + M.box[d]);
+    for (int side = 0; side < 2; ++side) {
+      if (rc[side].hi - rc[side].lo <= direct) {
+        add_segments(rc[side], order, pts, ctr, M);
+        continue;
+      }
+      const Mom64 &C = m64[cid[side]];
+      double r[3];
+      for (int d = 0; d < 3; ++d) r[d] = 0.5 * (cb[side][3 + d] + cb[side][d]) - ctr[d];
+      double cc[3];
+      for (int i = 0; i < 3; ++i) cc[i] = C.c[i];
+      for (int kk = 0; kk < 3; ++kk) {
+        double dk[3];
+        for (int j = 0; j < 3; ++j) dk[j] = C.D[3 * kk + j];
+        for (int i = 0; i < 3; ++i)
+          for (int j = i; j < 3; ++j)
+            M.Q[6 * kk + qidx(i, j)] += C.Q[6 * kk + qidx(i, j)] + r[j] * dk[i] + r[i] * dk[j] +
+                                        cc[kk] * (r[i] * r[j]);
+        for (int j = 0; j < 3; ++j) M.D[3 * kk + j] += dk[j] + cc[kk] * r[j];
+        M.c[kk] += cc[kk];
+      }
+    }
+    m64[id] = M;
+  }
+  store_node(M, ch, rr, geo, mom, id);
 }
 
 // ---------------------------------------------------------------- far field
