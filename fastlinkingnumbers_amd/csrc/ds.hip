@@ -228,25 +228,25 @@ __global__ __launch_bounds__(256) void ds_tile_f32(
         xp[k] = fmaf(div1, div2, -dt[k] * dt[k]);
         s12[k] = div1 + div2;
         yp[k] = dt[k] * s12[k];
-        // small angle: x' > 0, no branch correction, |y'/x'| <= 1/8 (per-pair
-        // tests interleave with the arithmetic; a max/min fold at the end of
-        // the step serialises and measured 5-9% slower)
-        small = small && (s12[k] > 0.f) && (fmaf(0.125f, xp[k], -fabsf(yp[k])) > 0.f);
+        // small angle: no branch correction, |y'/x'| <= 1/8 and x' > 8e-18
+        // (per-pair tests interleave with the arithmetic; a max/min fold at
+        // the end of the step serialises and measured 5-9% slower)
+        small = small && (s12[k] > 0.f) && (fmaf(0.125f, xp[k], -fabsf(yp[k])) > 1e-18f);
       }
       // Two small angles add as one: arg(z1) + arg(z2) = arg(z1 z2) with
       // z = x' + i y', so a couple of pairs needs one reciprocal (the
       // transcendental unit is the scarce one).  Both |y'/x'| <= 1/8 keep the
       // product's real part above 63/64 x1 x2 > 0 and its tangent within
-      // 16/63; the product must stay clear of FP32 underflow.
+      // 16/63, and both x' > 8e-18 keep it above 6e-35, clear of FP32
+      // underflow.
       float zr[R / 2], zi[R / 2];
 #pragma unroll
       for (int c = 0; c < R / 2; ++c) {
         zr[c] = fmaf(xp[2 * c], xp[2 * c + 1], -yp[2 * c] * yp[2 * c + 1]);
         zi[c] = fmaf(xp[2 * c], yp[2 * c + 1], yp[2 * c] * xp[2 * c + 1]);
-        small = small && (zr[c] > 1e-30f);
       }
       // every lane on the short path: a scalar test of the lane mask
-      if (__builtin_amdgcn_ballot_w64(!small) == 0) {
+      if (__builtin_expect(__builtin_amdgcn_ballot_w64(!small) == 0, 1)) {
         // atan(t) = t P(t^2) on |t| <= 16/63: minimax coefficients fitted for
         // this engine (relative error 2e-10 in exact arithmetic, no bias)
 #pragma unroll
