@@ -184,7 +184,8 @@ __global__ __launch_bounds__(256) void ds_tile_f32(
     // One column step: the new column (j+1), then the R segment pairs (k, j)
     // of the quadrilaterals a=G(k,j) b=G(k+1,j) c=G(k+1,j+1) d=G(k,j+1).
     auto step = [&](int j, const float (&gn)[R + 1], const float (&gl)[R + 1], const float (&gcol)[R],
-                    float (&nn)[R + 1], float (&nl)[R + 1], float (&ncol)[R], float &accf, int &corr) {
+                    float (&nn)[R + 1], float (&nl)[R + 1], float (&ncol)[R], float &accf, float &sacc,
+                    int &corr) {
       const float4 q = nq, ql = nql, e2 = ne;
       {
         const int w = min(j + 2, ncl);
@@ -260,9 +261,11 @@ __global__ __launch_bounds__(256) void ds_tile_f32(
           accf = fmaf(tt, p, accf);
         }
       } else {
+        // the general path keeps its own accumulators, so the frequent short
+        // path merges without register copies
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-          accf += atan2_fast(yp[k], xp[k]);
+          sacc += atan2_fast(yp[k], xp[k]);
           corr += (s12[k] < 0.f && dt[k] != 0.f) ? (dt[k] > 0.f ? 1 : -1) : 0;
         }
       }
@@ -274,13 +277,13 @@ __global__ __launch_bounds__(256) void ds_tile_f32(
     int j = tk.col0;
     while (j < jend) {
       const int jrun = min(jend, j + 64);
-      float accf = 0.f;
+      float accf = 0.f, sacc = 0.f;
       for (; j + 1 < jrun; j += 2) {
-        step(j, an, al, acol, bn, bl, bcol, accf, corr);
-        step(j + 1, bn, bl, bcol, an, al, acol, accf, corr);
+        step(j, an, al, acol, bn, bl, bcol, accf, sacc, corr);
+        step(j + 1, bn, bl, bcol, an, al, acol, accf, sacc, corr);
       }
       if (j < jrun) {
-        step(j, an, al, acol, bn, bl, bcol, accf, corr);
+        step(j, an, al, acol, bn, bl, bcol, accf, sacc, corr);
 #pragma unroll
         for (int k = 0; k <= R; ++k) {
           an[k] = bn[k]; al[k] = bl[k];
@@ -289,7 +292,7 @@ __global__ __launch_bounds__(256) void ds_tile_f32(
         for (int k = 0; k < R; ++k) acol[k] = bcol[k];
         ++j;
       }
-      acc += (double)accf;
+      acc += (double)accf + (double)sacc;
     }
     acc = wave_sum(acc);
     corr = wave_sum(corr);
