@@ -69,6 +69,11 @@ __device__ __forceinline__ uint64_t spread16(uint64_t v) {
 
 // One thread per segment of the Barnes–Hut curves: key = curve rank << 48 |
 // Morton code of the midpoint in the curve's box.
+// Morton resolution per axis, relative to each curve's own box: 10 bits keep
+// the sort keys at 30 + log2(curves) bits (6 radix passes for 10,000
+// curves); segments sharing a cell keep their order along the curve.
+constexpr int kMortonAxisBits = 10;
+
 __global__ __launch_bounds__(256) void bh_keys(const double4 *__restrict__ pts,
                                                const int64_t *__restrict__ seg_off,
                                                const int64_t *__restrict__ vtx_off,
@@ -92,10 +97,11 @@ __global__ __launch_bounds__(256) void bh_keys(const double4 *__restrict__ pts,
     const double ext = cbox[6 * c + 3 + d] - cbox[6 * c + d];
     double u = ext > 0 ? (m[d] - cbox[6 * c + d]) / ext : 0.0;
     u = fmin(fmax(u, 0.0), 1.0);
-    const uint64_t qd = (uint64_t)fmin(u * 65536.0, 65535.0);
+    const double cells = (double)(1 << kMortonAxisBits);
+    const uint64_t qd = (uint64_t)fmin(u * cells, cells - 1.0);
     code |= spread16(qd) << (2 - d);
   }
-  keys[g] = ((uint64_t)c << 48) | code;
+  keys[g] = ((uint64_t)c << (3 * kMortonAxisBits)) | code;
   vals[g] = (uint32_t)v;
 }
 
@@ -903,8 +909,8 @@ void build_forest(flk_ctx *ctx, const std::vector<int32_t> &curves_in, int leaf,
                      cv.dpts.as<double4>(), d_seg.as<int64_t>(), d_vtx.as<int64_t>(),
                      d_box.as<double>(), nc, S, d_k0.as<uint64_t>(), d_v0.as<uint32_t>());
   FLK_HIP(hipGetLastError());
-  int end_bit = 48;
-  while ((1LL << (end_bit - 48)) < nc) ++end_bit;
+  int end_bit = 3 * kMortonAxisBits;
+  while ((1LL << (end_bit - 3 * kMortonAxisBits)) < nc) ++end_bit;
   size_t tb = 0;
   FLK_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, d_k0.as<uint64_t>(), d_k1.as<uint64_t>(),
                                              d_v0.as<uint32_t>(), F.order.as<uint32_t>(), (int)S, 0,
