@@ -198,13 +198,14 @@ __global__ __launch_bounds__(256) void ds_tile_f32(
       float row[R + 1], diag[R];
       // G(k,j).G(k,j+1) and G(k,j+1).G(k+1,j+1) by polarization (e2.w, he1:
       // -|edge|^2 / 2), the diagonal G(k,j).G(k+1,j+1) as G(k,j+1).G(k+1,j+1)
-      // - e2.G(k+1,j+1)
+      // - e2.G(k+1,j+1), where e2.G(k+1,j+1) = |G(k+1,j+1)|^2 - G(k+1,j).G(k+1,j+1)
+      // since G(k+1,j) = G(k+1,j+1) - e2 (two operations instead of three)
 #pragma unroll
       for (int k = 0; k <= R; ++k) row[k] = fmaf(0.5f, gn[k] + nn[k], e2.w);
 #pragma unroll
       for (int k = 0; k < R; ++k) {
         ncol[k] = fmaf(0.5f, nn[k] + nn[k + 1], he1[k]);
-        diag[k] = fmaf(-e2.x, nx[k + 1], fmaf(-e2.y, ny[k + 1], fmaf(-e2.z, nz[k + 1], ncol[k])));
+        diag[k] = ncol[k] + (row[k + 1] - nn[k + 1]);
       }
       // det of the couple (2c, 2c+1) from one cross product at grid point
       // 2c+1 (computed for all couples first: interleaving them with the
