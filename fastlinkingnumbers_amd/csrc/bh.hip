@@ -589,13 +589,17 @@ __device__ __forceinline__ int wave_lds_slot(bool pred, int *counter) {
 // coalesced behind one queue atomic each.
 template <int IPT>
 __global__ __launch_bounds__(kTravBlock) void bh_level(
-    const BhItem *__restrict__ items, int64_t n, const BhGeo *__restrict__ geo,
-    const BhMom *__restrict__ mom, const float *__restrict__ beta2, float eps,
-    const float *__restrict__ pred, float log_tol,
+    const BhItem *__restrict__ items, int64_t n, const unsigned long long *__restrict__ n_dev,
+    const BhGeo *__restrict__ geo, const BhMom *__restrict__ mom, const float *__restrict__ beta2,
+    float eps, const float *__restrict__ pred, float log_tol,
     BhItem *__restrict__ next, unsigned long long *__restrict__ next_count, int64_t next_cap,
     BhItem *__restrict__ leaves, unsigned long long *__restrict__ leaf_count, int64_t leaf_cap,
     double *__restrict__ psum, double *__restrict__ perr) {
   constexpr int kTravItems = kTravBlock * IPT;
+  // A level enqueued before its count reached the host reads the count from
+  // the previous level's queue counter; its grid covers the bound n.
+  if (n_dev) n = min(n, (int64_t)*n_dev);
+  if ((int64_t)blockIdx.x * kTravItems >= n) return;  // whole block: before any barrier
   __shared__ int4 stage[kTravItems];
   __shared__ TravShared sh;
   if (threadIdx.x == 0) sh.nsplit = sh.nleaf = 0;
@@ -1053,6 +1057,7 @@ void traverse(flk_ctx *ctx, Forest &F, const std::vector<int32_t> &ra, const std
   if (np == 0) return;
   constexpr int64_t kChunk = 1 << 24;
   constexpr int kSlots = 1024;  // count pairs per zeroing of the counter array
+  constexpr int kRound = 4;     // levels per host round trip (at most)
   // items per thread of the level/leaf kernels: 2, 4 or 8 (grid and kernel agree)
   const int ipt = (ctx->tune.bh_items_per_thread == 2 || ctx->tune.bh_items_per_thread == 8)
                       ? ctx->tune.bh_items_per_thread
@@ -1061,7 +1066,7 @@ void traverse(flk_ctx *ctx, Forest &F, const std::vector<int32_t> &ra, const std
   B.roots_b.reserve(sizeof(int32_t) * np);
   B.beta2.reserve(sizeof(float) * np);
   B.cnt.reserve(sizeof(unsigned long long) * (4 + 2 * kSlots));
-  if (!B.host_cnt) FLK_HIP(hipHostMalloc((void **)&B.host_cnt, 2 * sizeof(unsigned long long)));
+  if (!B.host_cnt) FLK_HIP(hipHostMalloc((void **)&B.host_cnt, 2 * kRound * sizeof(unsigned long long)));
   FLK_HIP(hipMemcpyAsync(B.roots_a.ptr, ra.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice, s));
   FLK_HIP(hipMemcpyAsync(B.roots_b.ptr, rb.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice, s));
   FLK_HIP(hipMemcpyAsync(B.beta2.ptr, beta2.data(), sizeof(float) * np, hipMemcpyHostToDevice, s));
@@ -1090,7 +1095,8 @@ void traverse(flk_ctx *ctx, Forest &F, const std::vector<int32_t> &ra, const std
     B.pred.reserve(sizeof(float) * std::min(np, kChunk));
   }
   unsigned long long visits = 0, far = 0;
-  int64_t iter = 0;
+  int slot_next = 0;
+  DevBuf *alt = nullptr;  // second output buffer of a multi-level round
   while (!stack.empty()) {
     Level &top = stack.back();
     if (top.n == 0) {
@@ -1101,30 +1107,52 @@ void traverse(flk_ctx *ctx, Forest &F, const std::vector<int32_t> &ra, const std
     const int64_t take = std::min<int64_t>(top.n, kChunk);
     top.n -= take;
     const BhItem *src = top.buf->as<BhItem>() + top.n;
-    spare->reserve(sizeof(BhItem) * 2 * take);
-    if (B.leaves.bytes < sizeof(BhItem) * take) {
-      FLK_HIP(hipStreamSynchronize(s));  // the previous leaf launch may still read it
-      B.leaves.reserve(sizeof(BhItem) * take);
+    // Levels enqueued before the next read-back: the first from its exact
+    // count and, when it is consumed whole, up to kRound - 1 more, level d
+    // sized for its bound take * 2^d (a level at most doubles) and reading
+    // its true count from the previous level's counter on the device.  The
+    // last level's bound keeps its output within 2 * kChunk items.  With
+    // per-launch timing every level is its own round.
+    int D = 1;
+    if (!timing && top.n == 0)
+      while (D < kRound && (take << (D + 1)) <= 2 * kChunk) ++D;
+    const int64_t out_cap = take << D;  // items: bound of the last level's output
+    spare->reserve(sizeof(BhItem) * out_cap);
+    if (D > 1) {
+      if (!alt) alt = level_buf(out_cap);
+      alt->reserve(sizeof(BhItem) * out_cap);
     }
-    if (use_mlp) B.pred.reserve(sizeof(float) * take);
-    const int slot = (int)(iter % kSlots);
-    if (slot == 0 && iter > 0)
+    const int64_t leaf_cap = (int64_t)((1LL << D) - 1) * take;  // sum of the level bounds
+    if (B.leaves.bytes < sizeof(BhItem) * leaf_cap) {
+      FLK_HIP(hipStreamSynchronize(s));  // the previous leaf launch may still read it
+      B.leaves.reserve(sizeof(BhItem) * leaf_cap);
+    }
+    if (use_mlp) B.pred.reserve(sizeof(float) * (take << (D - 1)));
+    if (slot_next + D > kSlots) {
       FLK_HIP(hipMemsetAsync(cnt + 4, 0, sizeof(unsigned long long) * 2 * kSlots, s));
-    if (use_mlp)
-      mlp_predict_items(ctx, src, take, F.geo.as<BhGeo>(), eps, B.pred.as<float>(), timing ? ctx->ev0 : nullptr,
-                        timing ? ctx->ev3 : nullptr);
-    unsigned long long *qc = cnt + 4 + 2 * slot;  // [0] next items, [1] leaf items
-    // with timing, start/stop events ride on the dispatch itself
-    {
-      const dim3 grid((unsigned)((take + ipt * kTravBlock - 1) / (ipt * kTravBlock)));
+      slot_next = 0;
+    }
+    unsigned long long *qc = cnt + 4 + 2 * slot_next;  // per level [0] next items; [1] of level 0: leaves
+    slot_next += D;
+    DevBuf *outb[kRound];
+    for (int d = 0; d < D; ++d) {
+      outb[d] = (d % 2 == 0) ? spare : alt;
+      const BhItem *in = d == 0 ? src : outb[d - 1]->as<BhItem>();
+      const int64_t bound = take << d;
+      const unsigned long long *n_dev = d == 0 ? nullptr : qc + 2 * (d - 1);
+      if (use_mlp)
+        mlp_predict_items(ctx, in, bound, n_dev, F.geo.as<BhGeo>(), eps, B.pred.as<float>(),
+                          timing ? ctx->ev0 : nullptr, timing ? ctx->ev3 : nullptr);
+      // with timing, start/stop events ride on the dispatch itself
+      const dim3 grid((unsigned)((bound + ipt * kTravBlock - 1) / (ipt * kTravBlock)));
       auto lev = ipt == 2 ? bh_level<2> : ipt == 8 ? bh_level<8> : bh_level<4>;
       launch_timed(timing ? ctx->ev1 : nullptr, timing ? ctx->ev2 : nullptr, lev, grid, dim3(kTravBlock), s,
-                   src, take, F.geo.as<BhGeo>(), F.mom.as<BhMom>(), (const float *)B.beta2.as<float>(), eps,
-                   (const float *)(use_mlp ? B.pred.as<float>() : nullptr), log_tol, spare->as<BhItem>(), qc,
-                   2 * take, B.leaves.as<BhItem>(), qc + 1, take, d_sum, d_err);
+                   in, bound, n_dev, F.geo.as<BhGeo>(), F.mom.as<BhMom>(), (const float *)B.beta2.as<float>(),
+                   eps, (const float *)(use_mlp ? B.pred.as<float>() : nullptr), log_tol, outb[d]->as<BhItem>(),
+                   qc + 2 * d, 2 * bound, B.leaves.as<BhItem>(), qc + 1, leaf_cap, d_sum, d_err);
+      FLK_HIP(hipGetLastError());
     }
-    FLK_HIP(hipGetLastError());
-    FLK_HIP(hipMemcpyAsync(B.host_cnt, qc, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    FLK_HIP(hipMemcpyAsync(B.host_cnt, qc, 2 * D * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     FLK_HIP(hipStreamSynchronize(s));
     if (timing) {
       float ms_pred = 0.f, ms_level = 0.f;
@@ -1133,29 +1161,38 @@ void traverse(flk_ctx *ctx, Forest &F, const std::vector<int32_t> &ra, const std
       ctx->stats.ms_mlp += ms_pred;
       ctx->stats.ms_bh_level += ms_level;
     }
-    ++iter;
-    const unsigned long long nn[2] = {B.host_cnt[0], B.host_cnt[1]};
-    if ((int64_t)nn[0] > 2 * take || (int64_t)nn[1] > take)
-      throw Error(FLK_ERR_STATE, "Barnes-Hut level buffer overflow");
     // every item is exactly one of: leaf pair, far field, split (two children)
-    visits += (unsigned long long)take;
-    far += (unsigned long long)take - nn[0] / 2 - nn[1];
-    if (nn[1] > 0) {
+    int64_t in_n = take;
+    for (int d = 0; d < D; ++d) {
+      const unsigned long long nn = B.host_cnt[2 * d];
+      if ((int64_t)nn > 2 * (take << d)) throw Error(FLK_ERR_STATE, "Barnes-Hut level buffer overflow");
+      visits += (unsigned long long)in_n;
+      far += (unsigned long long)in_n - nn / 2;
+      in_n = (int64_t)nn;
+    }
+    const unsigned long long nleaf = B.host_cnt[1];
+    if ((int64_t)nleaf > leaf_cap) throw Error(FLK_ERR_STATE, "Barnes-Hut leaf buffer overflow");
+    far -= nleaf;
+    if (nleaf > 0) {
       const Curves &cv = ctx->curves;
       auto leafk = ipt == 2 ? bh_leaf_pairs<2> : ipt == 8 ? bh_leaf_pairs<8> : bh_leaf_pairs<4>;
-      hipLaunchKernelGGL(leafk, dim3((unsigned)((nn[1] + ipt * kTravBlock - 1) / (ipt * kTravBlock))),
-                         dim3(kTravBlock), 0, s, B.leaves.as<BhItem>(), (int64_t)nn[1], F.geo.as<BhGeo>(),
+      hipLaunchKernelGGL(leafk, dim3((unsigned)((nleaf + ipt * kTravBlock - 1) / (ipt * kTravBlock))),
+                         dim3(kTravBlock), 0, s, B.leaves.as<BhItem>(), (int64_t)nleaf, F.geo.as<BhGeo>(),
                          F.order.as<uint32_t>(), cv.hi.as<float4>(), cv.lo.as<float4>(),
                          cv.ed.as<float4>(), d_sum, d_err, cnt);
       FLK_HIP(hipGetLastError());
     }
-    if (nn[0] == 0) continue;
+    DevBuf *last = outb[D - 1];
     if (top.n == 0) {
-      // the level is consumed: its children replace it
-      std::swap(top.buf, spare);
-      top.n = (int64_t)nn[0];
-    } else {
-      stack.push_back(Level{spare, (int64_t)nn[0]});
+      // the level is consumed: the last level's output replaces it, the
+      // consumed buffer and the round's other output buffer are free
+      DevBuf *consumed = top.buf;
+      top.buf = last;
+      top.n = in_n;
+      if (D > 1) alt = (last == spare) ? alt : spare;
+      spare = consumed;
+    } else if (in_n > 0) {
+      stack.push_back(Level{last, in_n});  // D == 1: last is spare
       spare = level_buf(2 * kChunk);
     }
   }

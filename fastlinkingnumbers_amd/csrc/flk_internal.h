@@ -142,7 +142,7 @@ struct BhScratch {
   // traversal
   DevBuf roots_a, roots_b, beta2, cnt, leaves, pred, sum, err;
   std::vector<std::unique_ptr<DevBuf>> levels;
-  unsigned long long *host_cnt = nullptr;  // pinned, 2 words
+  unsigned long long *host_cnt = nullptr;  // pinned, 2 words per level of a traversal round trip
   template <typename Fn> void each(Fn fn) {
     DevBuf *all[] = {&geo, &mom, &order, &m64, &range, &child, &flag, &scan, &keys0, &keys1, &ids0,
                      &tmp, &seg, &vtx, &box, &roots_a, &roots_b, &beta2, &cnt, &leaves, &pred,
@@ -303,8 +303,9 @@ void mlp_forward_device(flk_ctx *ctx, const float *dx, int64_t n, float *dy);
 // +inf where the decision does not depend on it), from the node records.
 struct BhItem;
 struct BhGeo;
-void mlp_predict_items(flk_ctx *ctx, const BhItem *items, int64_t n, const BhGeo *geo, float eps,
-                       float *pred, hipEvent_t start, hipEvent_t stop);
+// n items, or with n_dev the count *n_dev (read on the device) bounded by n
+void mlp_predict_items(flk_ctx *ctx, const BhItem *items, int64_t n, const unsigned long long *n_dev,
+                       const BhGeo *geo, float eps, float *pred, hipEvent_t start, hipEvent_t stop);
 float mlp_gradient(flk_ctx *ctx, const float *x, const float *y, int64_t n, float *grad);
 float mlp_train(flk_ctx *ctx, const float *x, const float *y, int64_t n, int epochs, int batch,
                 float lr, uint64_t seed);

@@ -269,6 +269,7 @@ __global__ __launch_bounds__(256) void mlp_forward_kernel(const float *__restric
 // compacted into the wave's LDS rows and evaluated 16 per tile.  The next
 // iteration's items and node records are loaded before the tiles run.
 __global__ __launch_bounds__(256) void bh_mlp_predict(const BhItem *__restrict__ items, int64_t n,
+                                                      const unsigned long long *__restrict__ n_dev,
                                                       const BhGeo *__restrict__ geo, float eps,
                                                       const float *__restrict__ P,
                                                       const float *__restrict__ norm,
@@ -278,6 +279,7 @@ __global__ __launch_bounds__(256) void bh_mlp_predict(const BhItem *__restrict__
   __shared__ int32_t src[4][64];
   InferRegs R;
   load_infer(P, norm, R, L);
+  if (n_dev) n = min(n, (int64_t)*n_dev);  // level enqueued ahead: n is the bound
   const int l = threadIdx.x & 63, w = threadIdx.x >> 6, g = l >> 4, c = l & 15;
   const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + w;
   const int64_t stride = (int64_t)gridDim.x * (blockDim.x >> 6) * 64;
@@ -540,11 +542,11 @@ void mlp_forward_device(flk_ctx *ctx, const float *dx, int64_t n, float *dy) {
   FLK_HIP(hipGetLastError());
 }
 
-void mlp_predict_items(flk_ctx *ctx, const BhItem *items, int64_t n, const BhGeo *geo, float eps,
-                       float *pred, hipEvent_t start, hipEvent_t stop) {
+void mlp_predict_items(flk_ctx *ctx, const BhItem *items, int64_t n, const unsigned long long *n_dev,
+                       const BhGeo *geo, float eps, float *pred, hipEvent_t start, hipEvent_t stop) {
   if (n <= 0) return;
   const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8 * std::max(1, ctx->num_cus)));
-  launch_timed(start, stop, bh_mlp_predict, dim3(blocks), dim3(256), ctx->stream, items, n, geo, eps,
+  launch_timed(start, stop, bh_mlp_predict, dim3(blocks), dim3(256), ctx->stream, items, n, n_dev, geo, eps,
                ctx->mlp.d_params.as<float>(), ctx->mlp.d_norm.as<float>(), pred);
   FLK_HIP(hipGetLastError());
 }
