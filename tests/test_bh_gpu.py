@@ -122,7 +122,10 @@ def test_bh_lattice_hybrid_batches(engine):
 @pytest.mark.parametrize("criterion", ["beta", "mlp"])
 def test_bh_kernel_timing_mode(engine, criterion):
     """params.timing = 1 times every level (and prediction) launch with HIP
-    events without changing what is computed."""
+    events without changing what is computed.  A timed traversal also reads
+    the queue counts back after every level, while the plain one enqueues up
+    to four levels per read-back with device-side counts: the two schedules
+    must visit, accept and evaluate exactly the same node pairs."""
     from fastlinkingnumbers_amd import criterion as crit
     engine.mlp_set_weights(*crit.load_weights())
     curves = W.random_walk_pair(seed=3, n=50000, rg=40.0, workload=3)
@@ -132,5 +135,6 @@ def test_bh_kernel_timing_mode(engine, criterion):
     assert plain.stats["ms_bh_level"] == 0.0 and plain.stats["ms_mlp"] == 0.0
     assert 0.0 < timed.stats["ms_bh_level"] < timed.stats["ms_total"]
     assert (timed.stats["ms_mlp"] > 0.0) == (criterion == "mlp")
-    assert timed.stats["bh_visits"] == plain.stats["bh_visits"]
+    for key in ("bh_visits", "bh_far", "bh_leaf_pairs"):
+        assert timed.stats[key] == plain.stats[key], key
     np.testing.assert_allclose(timed.lk, plain.lk, rtol=0, atol=1e-9)
