@@ -178,9 +178,7 @@ __device__ void add_segments(BhRange r, const uint32_t *__restrict__ order,
 
 // FP64 moments -> FP32 node records (centre hi/lo, radius, children, segment
 // range, moment norms for the error estimate; c, D, Q).
-__device__ void store_node(const Mom64 &M, int2 ch, BhRange r, BhGeo *__restrict__ geo,
-                           BhMom *__restrict__ mom, int id) {
-  BhGeo g;
+__device__ __forceinline__ void node_records(const Mom64 &M, int2 ch, BhRange r, BhGeo &g, BhMom &m) {
   double ctr[3], dsq = 0;
   for (int d = 0; d < 3; ++d) {
     ctr[d] = 0.5 * (M.box[3 + d] + M.box[d]);
@@ -207,13 +205,10 @@ __device__ void store_node(const Mom64 &M, int2 ch, BhRange r, BhGeo *__restrict
   g.dn = (float)(sqrt(dn) * 0.57735026918962576451);
   g.qn = (float)((1.0 / 3.0) * sqrt(qn));
   g.pad = 0.f;
-  geo[id] = g;
-  BhMom m;
   for (int t = 0; t < 3; ++t) m.c[t] = (float)M.c[t];
   for (int t = 0; t < 9; ++t) m.D[t] = (float)M.D[t];
   for (int t = 0; t < 18; ++t) m.Q[t] = (float)M.Q[t];
   m.pad[0] = m.pad[1] = 0.f;
-  mom[id] = m;
 }
 
 // Moments and records of a node of at most `direct` segments, from its
@@ -222,9 +217,8 @@ __device__ void store_node(const Mom64 &M, int2 ch, BhRange r, BhGeo *__restrict
 // (at most `direct` terms, the precision of the FP32 records they end in).
 // Both moment kernels use it, so a node's records depend only on its own
 // segments, never on which launch computed them.
-__device__ void direct_node(BhRange rr, int2 ch, const uint32_t *__restrict__ order,
-                            const double4 *__restrict__ pts, BhGeo *__restrict__ geo, BhMom *__restrict__ mom,
-                            int id) {
+__device__ __forceinline__ void direct_records(BhRange rr, int2 ch, const uint32_t *__restrict__ order,
+                                               const double4 *__restrict__ pts, BhGeo &gr, BhMom &m) {
   double box[6];
   leaf_box(rr, order, pts, box);
   double ctr[3], dsq = 0;
@@ -233,7 +227,6 @@ __device__ void direct_node(BhRange rr, int2 ch, const uint32_t *__restrict__ or
     const double e = box[3 + d] - box[d];
     dsq += e * e;
   }
-  BhMom m;
   for (int t = 0; t < 3; ++t) m.c[t] = 0.f;
   for (int t = 0; t < 9; ++t) m.D[t] = 0.f;
   for (int t = 0; t < 18; ++t) m.Q[t] = 0.f;
@@ -256,7 +249,6 @@ __device__ void direct_node(BhRange rr, int2 ch, const uint32_t *__restrict__ or
               fmaf(t[kk], fmaf((1.0f / 12.0f) * t[i], t[j], w[i] * w[j]), m.Q[6 * kk + qidx(i, j)]);
   }
   m.pad[0] = m.pad[1] = 0.f;
-  BhGeo gr;
   const float hx = (float)ctr[0], hy = (float)ctr[1], hz = (float)ctr[2];
   gr.cx = hx; gr.cy = hy; gr.cz = hz;
   gr.lx = (float)(ctr[0] - hx); gr.ly = (float)(ctr[1] - hy); gr.lz = (float)(ctr[2] - hz);
@@ -277,49 +269,91 @@ __device__ void direct_node(BhRange rr, int2 ch, const uint32_t *__restrict__ or
   gr.dn = sqrtf(dn) * 0.57735026918962576451f;
   gr.qn = (1.0f / 3.0f) * sqrtf(qn);
   gr.pad = 0.f;
-  geo[id] = gr;
-  mom[id] = m;
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
+// Stores the FP32 records of a wave's nw consecutive nodes (lane l holds
+// node l's; geo and mom point at the wave's first node) through the wave's
+// LDS buffer sw (at least 256 float4), so each store instruction writes 1 KB
+// of contiguous memory.
+__device__ __forceinline__ void wave_store_records(const BhGeo &gr, const BhMom &m, BhGeo *__restrict__ geo,
+                                                   BhMom *__restrict__ mom, int nw, float4 *sw) {
+  const int lane = threadIdx.x & 63;
+  const float4 *g4 = reinterpret_cast<const float4 *>(&gr);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) sw[4 * lane + i] = g4[i];
+  wave_lds_sync();
+  float4 *gdst = reinterpret_cast<float4 *>(geo);
+  for (int c = lane; c < 4 * nw; c += 64) gdst[c] = sw[c];
+  wave_lds_sync();
+  const float4 *m4 = reinterpret_cast<const float4 *>(&m);
+  float4 *mdst = reinterpret_cast<float4 *>(mom);
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {  // records of lanes 32h .. 32h + 31
+    if ((lane >> 5) == h)
+#pragma unroll
+      for (int i = 0; i < 8; ++i) sw[8 * (lane & 31) + i] = m4[i];
+    wave_lds_sync();
+    const int lim = 8 * min(32, nw - 32 * h);
+    for (int c = lane; c < lim; c += 64) mdst[256 * h + c] = sw[c];
+    wave_lds_sync();
+  }
+}
+
+// The same for FP64 records, kept only by the lanes in `has` (others skip
+// theirs): 16 lanes' records (288 float4) per pass through sw.
+__device__ __forceinline__ void wave_store_m64(const Mom64 &M, unsigned long long has, Mom64 *__restrict__ dst,
+                                               int nw, float4 *sw) {
+  constexpr int kChunks = (int)(sizeof(Mom64) / 16);  // 18
+  static_assert(sizeof(Mom64) % 16 == 0, "whole 16-byte chunks");
+  const int lane = threadIdx.x & 63;
+  const float4 *src = reinterpret_cast<const float4 *>(&M);
+  float4 *d = reinterpret_cast<float4 *>(dst);
+#pragma unroll
+  for (int h = 0; h < 4; ++h) {  // lanes 16h .. 16h + 15
+    if ((has >> (16 * h)) & 0xffffull) {
+      if ((lane >> 4) == h && ((has >> lane) & 1ull))
+#pragma unroll
+        for (int i = 0; i < kChunks; ++i) sw[kChunks * (lane & 15) + i] = src[i];
+      wave_lds_sync();
+      for (int c = lane; c < kChunks * 16; c += 64) {
+        const int owner = 16 * h + c / kChunks;
+        if (owner < nw && ((has >> owner) & 1ull)) d[kChunks * 16 * h + c] = sw[c];
+      }
+      wave_lds_sync();
+    }
+  }
 }
 
 // The bottom levels, where every node has at most `direct` segments, in one
 // launch over all of them; registers stay low enough for full occupancy.
+// A wave's 64 consecutive records leave through a 4 KB LDS buffer, so every
+// store instruction writes 1 KB of contiguous memory instead of one 16-byte
+// piece of each lane's own record.
 __global__ __launch_bounds__(256) void bh_moments_direct(const BhRange *__restrict__ range, int32_t base,
                                                          int32_t n, const int2 *__restrict__ child,
                                                          const uint32_t *__restrict__ order,
                                                          const double4 *__restrict__ pts, BhGeo *__restrict__ geo,
                                                          BhMom *__restrict__ mom) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  const int id = base + k;
-  direct_node(range[id], child[id], order, pts, geo, mom, id);
+  __shared__ float4 stage[4][256];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int k0 = blockIdx.x * blockDim.x + w * 64;  // this wave's first node past base
+  if (k0 >= n) return;                                // whole wave: no LDS exchange to join
+  const int k = k0 + lane, nw = min(64, n - k0);
+  BhGeo gr;
+  BhMom m;
+  if (k < n) direct_records(range[base + k], child[base + k], order, pts, gr, m);
+  wave_store_records(gr, m, geo + base + k0, mom + base + k0, nw, stage[w]);
 }
 
-// Bottom-up moments of one level (children already done) and the node's FP32
-// records.  A node of at most `direct` segments (leaf buckets and the small
-// subtrees above them, which are most of the nodes) is direct_node's and keeps
-// no FP64 record; a larger node adds
-// such a child's segments about its own centre directly (the closed form
-// shifted exactly) and a larger child's FP64 record with the shift formulas
-// of the reference (compute_link.cpp:358-389, Eqs. 21-23).  Only nodes above
-// `direct` segments write FP64 records, so the bottom levels never round-trip
-// them through memory.  Children are read one at a time.
-__global__ __launch_bounds__(128) void bh_moments(const BhRange *__restrict__ range, int32_t n,
-                                                  int32_t base, int32_t direct,
-                                                  const int2 *__restrict__ child,
-                                                  const uint32_t *__restrict__ order,
-                                                  const double4 *__restrict__ pts,
-                                                  Mom64 *__restrict__ m64, BhGeo *__restrict__ geo,
-                                                  BhMom *__restrict__ mom) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  const int id = base + k;
-  const int2 ch = child[id];
-  const BhRange rr = range[id];
-  if (ch.x < 0 || rr.hi - rr.lo <= direct) {
-    direct_node(rr, ch, order, pts, geo, mom, id);
-    return;
-  }
-  Mom64 M;
+// FP64 moments of a node above `direct` segments from its two children.
+__device__ void node_moments(BhRange rr, int2 ch, int32_t direct, const BhRange *__restrict__ range,
+                             const uint32_t *__restrict__ order, const double4 *__restrict__ pts,
+                             const Mom64 *__restrict__ m64, Mom64 &M) {
   for (int t = 0; t < 3; ++t) M.c[t] = 0;
   for (int t = 0; t < 9; ++t) M.D[t] = 0;
   for (int t = 0; t < 18; ++t) M.Q[t] = 0;
@@ -359,10 +393,50 @@ __global__ __launch_bounds__(128) void bh_moments(const BhRange *__restrict__ ra
         M.c[kk] += cc[kk];
       }
     }
-    m64[id] = M;
   }
-  store_node(M, ch, rr, geo, mom, id);
 }
+
+// Bottom-up moments of one level (children already done) and the node's FP32
+// records.  A node of at most `direct` segments (leaf buckets and the small
+// subtrees above them, which are most of the nodes) gets direct_records and
+// keeps no FP64 record; a larger node adds
+// such a child's segments about its own centre directly (the closed form
+// shifted exactly) and a larger child's FP64 record with the shift formulas
+// of the reference (compute_link.cpp:358-389, Eqs. 21-23).  Only nodes above
+// `direct` segments write FP64 records, so the bottom levels never round-trip
+// them through memory.  Children are read one at a time.
+__global__ __launch_bounds__(128) void bh_moments(const BhRange *__restrict__ range, int32_t n,
+                                                  int32_t base, int32_t direct,
+                                                  const int2 *__restrict__ child,
+                                                  const uint32_t *__restrict__ order,
+                                                  const double4 *__restrict__ pts,
+                                                  Mom64 *__restrict__ m64, BhGeo *__restrict__ geo,
+                                                  BhMom *__restrict__ mom) {
+  __shared__ float4 stage[2][18 * 16];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int k0 = blockIdx.x * blockDim.x + w * 64;  // this wave's first node past base
+  if (k0 >= n) return;                                // whole wave: no LDS exchange to join
+  const int k = k0 + lane, nw = min(64, n - k0);
+  BhGeo gr;
+  BhMom m;
+  Mom64 M;
+  bool fp64 = false;
+  if (k < n) {
+    const int id = base + k;
+    const int2 ch = child[id];
+    const BhRange rr = range[id];
+    if (ch.x < 0 || rr.hi - rr.lo <= direct) {
+      direct_records(rr, ch, order, pts, gr, m);
+    } else {
+      fp64 = true;
+      node_moments(rr, ch, direct, range, order, pts, m64, M);
+      node_records(M, ch, rr, gr, m);
+    }
+  }
+  wave_store_m64(M, __ballot(fp64), m64 + base + k0, nw, stage[w]);
+  wave_store_records(gr, m, geo + base + k0, mom + base + k0, nw, stage[w]);
+}
+
 
 // ---------------------------------------------------------------- far field
 struct F3 {
