@@ -138,16 +138,37 @@ __device__ __forceinline__ int qidx(int i, int j) {  // symmetric 3x3 -> 6
   return i <= j ? (i == 0 ? j : (i == 1 ? 2 + j : 5)) : (j == 0 ? i : (j == 1 ? 2 + i : 5));
 }
 
+// Endpoints of every segment in sorted order, gathered once after the sort
+// (six FP64 values per segment, copied unchanged from the vertex array), so
+// the moment kernels read each node's segments as one contiguous run.
+__global__ __launch_bounds__(256) void bh_stage_segments(const uint32_t *__restrict__ order,
+                                                         const double4 *__restrict__ pts, int64_t nseg,
+                                                         double2 *__restrict__ sp) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nseg) return;
+  const uint32_t v = order[s];
+  const double4 a = pts[v], b = pts[v + 1];
+  sp[3 * s] = make_double2(a.x, a.y);
+  sp[3 * s + 1] = make_double2(a.z, b.x);
+  sp[3 * s + 2] = make_double2(b.y, b.z);
+}
+
+__device__ __forceinline__ void seg_ends(const double2 *__restrict__ sp, int s, double (&pa)[3],
+                                         double (&pb)[3]) {
+  const double2 u = sp[3 * (int64_t)s], v = sp[3 * (int64_t)s + 1], w = sp[3 * (int64_t)s + 2];
+  pa[0] = u.x; pa[1] = u.y; pa[2] = v.x;
+  pb[0] = v.y; pb[1] = w.x; pb[2] = w.y;
+}
+
 // Bounding box of the segments [r.lo, r.hi) of a leaf bucket.
-__device__ void leaf_box(BhRange r, const uint32_t *__restrict__ order, const double4 *__restrict__ pts,
-                         double (&box)[6]) {
+__device__ void leaf_box(BhRange r, const double2 *__restrict__ sp, double (&box)[6]) {
   for (int t = 0; t < 3; ++t) {
     box[t] = DBL_MAX;
     box[3 + t] = -DBL_MAX;
   }
   for (int s = r.lo; s < r.hi; ++s) {
-    const double4 a = pts[order[s]], b = pts[order[s] + 1];
-    const double pa[3] = {a.x, a.y, a.z}, pb[3] = {b.x, b.y, b.z};
+    double pa[3], pb[3];
+    seg_ends(sp, s, pa, pb);
     for (int d = 0; d < 3; ++d) {
       box[d] = fmin(box[d], fmin(pa[d], pb[d]));
       box[3 + d] = fmax(box[3 + d], fmax(pa[d], pb[d]));
@@ -158,13 +179,13 @@ __device__ void leaf_box(BhRange r, const uint32_t *__restrict__ order, const do
 // Adds the closed-form moments of the segments [r.lo, r.hi) about ctr:
 // c += t, D += t (m - ctr)^T, Q_k += t_k (t t^T / 12 + (m-ctr)(m-ctr)^T)
 // (reference compute_link.cpp:333-356).
-__device__ void add_segments(BhRange r, const uint32_t *__restrict__ order,
-                             const double4 *__restrict__ pts, const double (&ctr)[3], Mom64 &M) {
+__device__ void add_segments(BhRange r, const double2 *__restrict__ sp, const double (&ctr)[3], Mom64 &M) {
   for (int s = r.lo; s < r.hi; ++s) {
-    const double4 a = pts[order[s]], b = pts[order[s] + 1];
-    const double t[3] = {b.x - a.x, b.y - a.y, b.z - a.z};
-    const double w[3] = {0.5 * (a.x + b.x) - ctr[0], 0.5 * (a.y + b.y) - ctr[1],
-                         0.5 * (a.z + b.z) - ctr[2]};
+    double a[3], b[3];
+    seg_ends(sp, s, a, b);
+    const double t[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    const double w[3] = {0.5 * (a[0] + b[0]) - ctr[0], 0.5 * (a[1] + b[1]) - ctr[1],
+                         0.5 * (a[2] + b[2]) - ctr[2]};
     for (int i = 0; i < 3; ++i) {
       M.c[i] += t[i];
       for (int j = 0; j < 3; ++j) M.D[3 * i + j] += t[i] * w[j];
@@ -217,10 +238,10 @@ __device__ __forceinline__ void node_records(const Mom64 &M, int2 ch, BhRange r,
 // (at most `direct` terms, the precision of the FP32 records they end in).
 // Both moment kernels use it, so a node's records depend only on its own
 // segments, never on which launch computed them.
-__device__ __forceinline__ void direct_records(BhRange rr, int2 ch, const uint32_t *__restrict__ order,
-                                               const double4 *__restrict__ pts, BhGeo &gr, BhMom &m) {
+__device__ __forceinline__ void direct_records(BhRange rr, int2 ch, const double2 *__restrict__ sp, BhGeo &gr,
+                                               BhMom &m) {
   double box[6];
-  leaf_box(rr, order, pts, box);
+  leaf_box(rr, sp, box);
   double ctr[3], dsq = 0;
   for (int d = 0; d < 3; ++d) {
     ctr[d] = 0.5 * (box[3 + d] + box[d]);
@@ -231,8 +252,8 @@ __device__ __forceinline__ void direct_records(BhRange rr, int2 ch, const uint32
   for (int t = 0; t < 9; ++t) m.D[t] = 0.f;
   for (int t = 0; t < 18; ++t) m.Q[t] = 0.f;
   for (int s = rr.lo; s < rr.hi; ++s) {
-    const double4 a = pts[order[s]], b = pts[order[s] + 1];
-    const double pa[3] = {a.x, a.y, a.z}, pb[3] = {b.x, b.y, b.z};
+    double pa[3], pb[3];
+    seg_ends(sp, s, pa, pb);
     float t[3], w[3];
     for (int d = 0; d < 3; ++d) {
       t[d] = (float)(pb[d] - pa[d]);
@@ -336,8 +357,7 @@ __device__ __forceinline__ void wave_store_m64(const Mom64 &M, unsigned long lon
 // piece of each lane's own record.
 __global__ __launch_bounds__(256) void bh_moments_direct(const BhRange *__restrict__ range, int32_t base,
                                                          int32_t n, const int2 *__restrict__ child,
-                                                         const uint32_t *__restrict__ order,
-                                                         const double4 *__restrict__ pts, BhGeo *__restrict__ geo,
+                                                         const double2 *__restrict__ sp, BhGeo *__restrict__ geo,
                                                          BhMom *__restrict__ mom) {
   __shared__ float4 stage[4][256];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -346,14 +366,13 @@ __global__ __launch_bounds__(256) void bh_moments_direct(const BhRange *__restri
   const int k = k0 + lane, nw = min(64, n - k0);
   BhGeo gr;
   BhMom m;
-  if (k < n) direct_records(range[base + k], child[base + k], order, pts, gr, m);
+  if (k < n) direct_records(range[base + k], child[base + k], sp, gr, m);
   wave_store_records(gr, m, geo + base + k0, mom + base + k0, nw, stage[w]);
 }
 
 // FP64 moments of a node above `direct` segments from its two children.
 __device__ void node_moments(BhRange rr, int2 ch, int32_t direct, const BhRange *__restrict__ range,
-                             const uint32_t *__restrict__ order, const double4 *__restrict__ pts,
-                             const Mom64 *__restrict__ m64, Mom64 &M) {
+                             const double2 *__restrict__ sp, const Mom64 *__restrict__ m64, Mom64 &M) {
   for (int t = 0; t < 3; ++t) M.c[t] = 0;
   for (int t = 0; t < 9; ++t) M.D[t] = 0;
   for (int t = 0; t < 18; ++t) M.Q[t] = 0;
@@ -362,7 +381,7 @@ __device__ void node_moments(BhRange rr, int2 ch, int32_t direct, const BhRange 
     const int cid[2] = {ch.x, ch.y};
     double cb[2][6];
     for (int side = 0; side < 2; ++side) {
-      if (rc[side].hi - rc[side].lo <= direct) leaf_box(rc[side], order, pts, cb[side]);
+      if (rc[side].hi - rc[side].lo <= direct) leaf_box(rc[side], sp, cb[side]);
       else
         for (int t = 0; t < 6; ++t) cb[side][t] = m64[cid[side]].box[t];
     }
@@ -374,7 +393,7 @@ __device__ void node_moments(BhRange rr, int2 ch, int32_t direct, const BhRange 
     for (int d = 0; d < 3; ++d) ctr[d] = 0.5 * (M.box[3 + d] + M.box[d]);
     for (int side = 0; side < 2; ++side) {
       if (rc[side].hi - rc[side].lo <= direct) {
-        add_segments(rc[side], order, pts, ctr, M);
+        add_segments(rc[side], sp, ctr, M);
         continue;
       }
       const Mom64 &C = m64[cid[side]];
@@ -408,8 +427,7 @@ __device__ void node_moments(BhRange rr, int2 ch, int32_t direct, const BhRange 
 __global__ __launch_bounds__(128) void bh_moments(const BhRange *__restrict__ range, int32_t n,
                                                   int32_t base, int32_t direct,
                                                   const int2 *__restrict__ child,
-                                                  const uint32_t *__restrict__ order,
-                                                  const double4 *__restrict__ pts,
+                                                  const double2 *__restrict__ sp,
                                                   Mom64 *__restrict__ m64, BhGeo *__restrict__ geo,
                                                   BhMom *__restrict__ mom) {
   __shared__ float4 stage[2][18 * 16];
@@ -426,10 +444,10 @@ __global__ __launch_bounds__(128) void bh_moments(const BhRange *__restrict__ ra
     const int2 ch = child[id];
     const BhRange rr = range[id];
     if (ch.x < 0 || rr.hi - rr.lo <= direct) {
-      direct_records(rr, ch, order, pts, gr, m);
+      direct_records(rr, ch, sp, gr, m);
     } else {
       fp64 = true;
-      node_moments(rr, ch, direct, range, order, pts, m64, M);
+      node_moments(rr, ch, direct, range, sp, m64, M);
       node_records(M, ch, rr, gr, m);
     }
   }
@@ -997,6 +1015,11 @@ void build_forest(flk_ctx *ctx, const std::vector<int32_t> &curves_in, int leaf,
   FLK_HIP(hipcub::DeviceRadixSort::SortPairs(d_tmp.ptr, tb, d_k0.as<uint64_t>(), d_k1.as<uint64_t>(),
                                              d_v0.as<uint32_t>(), F.order.as<uint32_t>(), (int)S, 0,
                                              end_bit, s));
+  DevBuf &d_sp = B.sp;
+  d_sp.reserve(sizeof(double2) * 3 * S);
+  hipLaunchKernelGGL(bh_stage_segments, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, s,
+                     F.order.as<uint32_t>(), cv.dpts.as<double4>(), S, d_sp.as<double2>());
+  FLK_HIP(hipGetLastError());
 
   // level-by-level node creation
   std::vector<int32_t> level_base{0}, level_n{nc};
@@ -1098,15 +1121,15 @@ void build_forest(flk_ctx *ctx, const std::vector<int32_t> &curves_in, int leaf,
   if (lev_top < (int)level_base.size()) {
     const int32_t b0 = level_base[lev_top], nd = total - b0;
     hipLaunchKernelGGL(bh_moments_direct, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, s,
-                       d_range.as<BhRange>(), b0, nd, d_child.as<int2>(), F.order.as<uint32_t>(),
-                       cv.dpts.as<double4>(), F.geo.as<BhGeo>(), F.mom.as<BhMom>());
+                       d_range.as<BhRange>(), b0, nd, d_child.as<int2>(), d_sp.as<double2>(),
+                       F.geo.as<BhGeo>(), F.mom.as<BhMom>());
     FLK_HIP(hipGetLastError());
   }
   for (int lev = lev_top - 1; lev >= 0; --lev) {
     const int32_t n = level_n[lev];
     hipLaunchKernelGGL(bh_moments, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, d_range.as<BhRange>(),
-                       n, level_base[lev], direct, d_child.as<int2>(), F.order.as<uint32_t>(),
-                       cv.dpts.as<double4>(), d_m64.as<Mom64>(), F.geo.as<BhGeo>(), F.mom.as<BhMom>());
+                       n, level_base[lev], direct, d_child.as<int2>(), d_sp.as<double2>(),
+                       d_m64.as<Mom64>(), F.geo.as<BhGeo>(), F.mom.as<BhMom>());
     FLK_HIP(hipGetLastError());
   }
   FLK_HIP(hipStreamSynchronize(s));
@@ -1444,8 +1467,9 @@ void bh_batch(flk_ctx *ctx, const int32_t *pairs, const std::vector<int64_t> &bh
 // Device bytes the trees and their build records take per segment: two
 // 192-byte node records (384), 288-byte FP64 records for the nodes above 8
 // segments (about one node in eight: ~72), ranges and children (32), sort keys,
-// ids and order (24), scan flags (8) and sort scratch, with headroom.
-constexpr double kBhBytesPerSegment = 600.0;
+// ids and order (24), the staged segment endpoints (48), scan flags (8) and
+// sort scratch, with headroom.
+constexpr double kBhBytesPerSegment = 648.0;
 
 }  // namespace
 

@@ -138,14 +138,14 @@ struct HostStage {
 // FP64 build records and the traversal queues.
 struct BhScratch {
   // forest
-  DevBuf geo, mom, order, m64, range, child, flag, scan, keys0, keys1, ids0, tmp, seg, vtx, box;
+  DevBuf geo, mom, order, m64, range, child, flag, scan, keys0, keys1, ids0, tmp, seg, vtx, box, sp;
   // traversal
   DevBuf roots_a, roots_b, beta2, cnt, leaves, pred, sum, err;
   std::vector<std::unique_ptr<DevBuf>> levels;
   unsigned long long *host_cnt = nullptr;  // pinned, 2 words per level of a traversal round trip
   template <typename Fn> void each(Fn fn) {
     DevBuf *all[] = {&geo, &mom, &order, &m64, &range, &child, &flag, &scan, &keys0, &keys1, &ids0,
-                     &tmp, &seg, &vtx, &box, &roots_a, &roots_b, &beta2, &cnt, &leaves, &pred,
+                     &tmp, &seg, &vtx, &box, &sp, &roots_a, &roots_b, &beta2, &cnt, &leaves, &pred,
                      &sum, &err};
     for (DevBuf *b : all) fn(*b);
     for (auto &b : levels) fn(*b);
