@@ -35,6 +35,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "bh_types.h"
 #include "flk_internal.h"
@@ -67,18 +68,17 @@ __device__ __forceinline__ uint64_t spread16(uint64_t v) {
   return v;
 }
 
-// One thread per segment of the Barnes–Hut curves: key = curve rank << 48 |
-// Morton code of the midpoint in the curve's box.
-// Morton resolution per axis, relative to each curve's own box: 10 bits keep
-// the sort keys at 30 + log2(curves) bits (6 radix passes for 10,000
-// curves); segments sharing a cell keep their order along the curve.
-constexpr int kMortonAxisBits = 10;
-
+// One thread per segment of the Barnes–Hut curves: key = curve rank <<
+// (3 * bits) | Morton code of the midpoint in the curve's box, `bits` per
+// axis.  cbox holds per curve the box corner (3 doubles) and cells / extent
+// per axis (0 for a flat axis).  Segments sharing a cell keep their order
+// along the curve (the sort is stable).
+template <typename K>
 __global__ __launch_bounds__(256) void bh_keys(const double4 *__restrict__ pts,
                                                const int64_t *__restrict__ seg_off,
                                                const int64_t *__restrict__ vtx_off,
-                                               const double *__restrict__ cbox, int32_t ncurves,
-                                               int64_t nseg, uint64_t *__restrict__ keys,
+                                               const double *__restrict__ cbox, int32_t ncurves, int bits,
+                                               int64_t nseg, K *__restrict__ keys,
                                                uint32_t *__restrict__ vals) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= nseg) return;
@@ -92,16 +92,14 @@ __global__ __launch_bounds__(256) void bh_keys(const double4 *__restrict__ pts,
   const int64_t v = vtx_off[c] + (g - seg_off[c]);
   const double4 p = pts[v], q = pts[v + 1];
   const double m[3] = {0.5 * (p.x + q.x), 0.5 * (p.y + q.y), 0.5 * (p.z + q.z)};
+  const double top = (double)((1 << bits) - 1);
   uint64_t code = 0;
   for (int d = 0; d < 3; ++d) {
-    const double ext = cbox[6 * c + 3 + d] - cbox[6 * c + d];
-    double u = ext > 0 ? (m[d] - cbox[6 * c + d]) / ext : 0.0;
-    u = fmin(fmax(u, 0.0), 1.0);
-    const double cells = (double)(1 << kMortonAxisBits);
-    const uint64_t qd = (uint64_t)fmin(u * cells, cells - 1.0);
+    const double u = (m[d] - cbox[6 * c + d]) * cbox[6 * c + 3 + d];
+    const uint64_t qd = (uint64_t)fmin(fmax(u, 0.0), top);
     code |= spread16(qd) << (2 - d);
   }
-  keys[g] = ((uint64_t)c << (3 * kMortonAxisBits)) | code;
+  keys[g] = (K)(((uint64_t)c << (3 * bits)) | code);
   vals[g] = (uint32_t)v;
 }
 
@@ -981,11 +979,23 @@ void build_forest(flk_ctx *ctx, const std::vector<int32_t> &curves_in, int leaf,
   for (int32_t k = 0; k < nc; ++k) {
     seg_off[k + 1] = seg_off[k] + cv.n[cs[k]];
     vtx_off[k] = cv.off[cs[k]];
-    for (int d = 0; d < 3; ++d) {
-      cbox[6 * k + d] = (cv.box[6 * cs[k] + d] - cv.origin[d]) * cv.scale;
-      cbox[6 * k + 3 + d] = (cv.box[6 * cs[k] + 3 + d] - cv.origin[d]) * cv.scale;
-    }
   }
+  // Sort keys: curve rank and a Morton code of 10 bits per axis relative to
+  // the curve's box (a curve's tree never depends on the batch around it);
+  // 4-byte keys when the rank fits the two bits left (at most 4 curves).
+  int curve_bits = 0;
+  while ((1LL << curve_bits) < nc) ++curve_bits;
+  const int axis_bits = 10;
+  const int end_bit = 3 * axis_bits + curve_bits;
+  const bool key32 = end_bit <= 32;
+  const double cells = (double)(1 << axis_bits);
+  for (int32_t k = 0; k < nc; ++k)
+    for (int d = 0; d < 3; ++d) {
+      const double lo = (cv.box[6 * cs[k] + d] - cv.origin[d]) * cv.scale;
+      const double hi = (cv.box[6 * cs[k] + 3 + d] - cv.origin[d]) * cv.scale;
+      cbox[6 * k + d] = lo;
+      cbox[6 * k + 3 + d] = hi > lo ? cells / (hi - lo) : 0.0;
+    }
   const int64_t S = seg_off[nc];
   if (S >= INT32_MAX / 2) throw Error(FLK_ERR_ARG, "too many Barnes-Hut segments");
   BhScratch &B = ctx->bh;
@@ -1001,20 +1011,21 @@ void build_forest(flk_ctx *ctx, const std::vector<int32_t> &curves_in, int leaf,
   FLK_HIP(hipMemcpyAsync(d_seg.ptr, seg_off.data(), sizeof(int64_t) * (nc + 1), hipMemcpyHostToDevice, s));
   FLK_HIP(hipMemcpyAsync(d_vtx.ptr, vtx_off.data(), sizeof(int64_t) * nc, hipMemcpyHostToDevice, s));
   FLK_HIP(hipMemcpyAsync(d_box.ptr, cbox.data(), sizeof(double) * 6 * nc, hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(bh_keys, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, s,
-                     cv.dpts.as<double4>(), d_seg.as<int64_t>(), d_vtx.as<int64_t>(),
-                     d_box.as<double>(), nc, S, d_k0.as<uint64_t>(), d_v0.as<uint32_t>());
-  FLK_HIP(hipGetLastError());
-  int end_bit = 3 * kMortonAxisBits;
-  while ((1LL << (end_bit - 3 * kMortonAxisBits)) < nc) ++end_bit;
-  size_t tb = 0;
-  FLK_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, d_k0.as<uint64_t>(), d_k1.as<uint64_t>(),
-                                             d_v0.as<uint32_t>(), F.order.as<uint32_t>(), (int)S, 0,
-                                             end_bit, s));
-  d_tmp.reserve(std::max<size_t>(tb, 16));
-  FLK_HIP(hipcub::DeviceRadixSort::SortPairs(d_tmp.ptr, tb, d_k0.as<uint64_t>(), d_k1.as<uint64_t>(),
-                                             d_v0.as<uint32_t>(), F.order.as<uint32_t>(), (int)S, 0,
-                                             end_bit, s));
+  auto sort_keys = [&](auto *k0, auto *k1) {
+    using K = std::remove_pointer_t<decltype(k0)>;
+    hipLaunchKernelGGL(bh_keys<K>, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, s,
+                       cv.dpts.as<double4>(), d_seg.as<int64_t>(), d_vtx.as<int64_t>(), d_box.as<double>(),
+                       nc, axis_bits, S, k0, d_v0.as<uint32_t>());
+    FLK_HIP(hipGetLastError());
+    size_t tb = 0;
+    FLK_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k1, d_v0.as<uint32_t>(), F.order.as<uint32_t>(),
+                                               (int)S, 0, end_bit, s));
+    d_tmp.reserve(std::max<size_t>(tb, 16));
+    FLK_HIP(hipcub::DeviceRadixSort::SortPairs(d_tmp.ptr, tb, k0, k1, d_v0.as<uint32_t>(), F.order.as<uint32_t>(),
+                                               (int)S, 0, end_bit, s));
+  };
+  if (key32) sort_keys(d_k0.as<uint32_t>(), d_k1.as<uint32_t>());
+  else sort_keys(d_k0.as<uint64_t>(), d_k1.as<uint64_t>());
   DevBuf &d_sp = B.sp;
   d_sp.reserve(sizeof(double2) * 3 * S);
   hipLaunchKernelGGL(bh_stage_segments, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, s,
@@ -1070,7 +1081,7 @@ void build_forest(flk_ctx *ctx, const std::vector<int32_t> &curves_in, int leaf,
   {
     int64_t widest = nc;
     for (int64_t sp : level_split) widest = std::max(widest, 2 * sp);
-    tb = 0;
+    size_t tb = 0;
     FLK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, d_flag.as<int32_t>(), d_scan.as<int32_t>(),
                                              (int)widest, s));
     FLK_HIP(hipStreamSynchronize(s));
