@@ -174,27 +174,6 @@ __device__ void leaf_box(BhRange r, const double2 *__restrict__ sp, double (&box
   }
 }
 
-// Adds the closed-form moments of the segments [r.lo, r.hi) about ctr:
-// c += t, D += t (m - ctr)^T, Q_k += t_k (t t^T / 12 + (m-ctr)(m-ctr)^T)
-// (reference compute_link.cpp:333-356).
-__device__ void add_segments(BhRange r, const double2 *__restrict__ sp, const double (&ctr)[3], Mom64 &M) {
-  for (int s = r.lo; s < r.hi; ++s) {
-    double a[3], b[3];
-    seg_ends(sp, s, a, b);
-    const double t[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
-    const double w[3] = {0.5 * (a[0] + b[0]) - ctr[0], 0.5 * (a[1] + b[1]) - ctr[1],
-                         0.5 * (a[2] + b[2]) - ctr[2]};
-    for (int i = 0; i < 3; ++i) {
-      M.c[i] += t[i];
-      for (int j = 0; j < 3; ++j) M.D[3 * i + j] += t[i] * w[j];
-    }
-    for (int kk = 0; kk < 3; ++kk)
-      for (int i = 0; i < 3; ++i)
-        for (int j = i; j < 3; ++j)
-          M.Q[6 * kk + qidx(i, j)] += t[kk] * ((1.0 / 12.0) * t[i] * t[j] + w[i] * w[j]);
-  }
-}
-
 // FP64 moments -> FP32 node records (centre hi/lo, radius, children, segment
 // range, moment norms for the error estimate; c, D, Q).
 __device__ __forceinline__ void node_records(const Mom64 &M, int2 ch, BhRange r, BhGeo &g, BhMom &m) {
@@ -232,7 +211,9 @@ __device__ __forceinline__ void node_records(const Mom64 &M, int2 ch, BhRange r,
 
 // Moments and records of a node of at most `direct` segments, from its
 // segments: the box and centre in FP64, each segment's tangent and centred
-// midpoint rounded to FP32 from FP64, and the closed forms accumulated in FP32
+// midpoint rounded to FP32 from FP64, and the closed forms (reference
+// compute_link.cpp:333-356: c += t, D += t (m - ctr)^T,
+// Q_k += t_k (t t^T / 12 + (m - ctr)(m - ctr)^T)) accumulated in FP32
 // (at most `direct` terms, the precision of the FP32 records they end in).
 // Both moment kernels use it, so a node's records depend only on its own
 // segments, never on which launch computed them.
@@ -370,7 +351,8 @@ __global__ __launch_bounds__(256) void bh_moments_direct(const BhRange *__restri
 
 // FP64 moments of a node above `direct` segments from its two children.
 __device__ void node_moments(BhRange rr, int2 ch, int32_t direct, const BhRange *__restrict__ range,
-                             const double2 *__restrict__ sp, const Mom64 *__restrict__ m64, Mom64 &M) {
+                             const double2 *__restrict__ sp, const Mom64 *__restrict__ m64,
+                             const BhMom *__restrict__ mom, Mom64 &M) {
   for (int t = 0; t < 3; ++t) M.c[t] = 0;
   for (int t = 0; t < 9; ++t) M.D[t] = 0;
   for (int t = 0; t < 18; ++t) M.Q[t] = 0;
@@ -389,14 +371,12 @@ __device__ void node_moments(BhRange rr, int2 ch, int32_t direct, const BhRange 
     }
     double ctr[3];
     for (int d = 0; d < 3; ++d) ctr[d] = 0.5 * (M.box[3 + d] + M.box[d]);
-    for (int side = 0; side < 2; ++side) {
-      if (rc[side].hi - rc[side].lo <= direct) {
-        add_segments(rc[side], sp, ctr, M);
-        continue;
-      }
-      const Mom64 &C = m64[cid[side]];
+    // a child's moments about its own box centre (an FP64 record, or the FP32
+    // record of a child of at most `direct` segments, written by an earlier
+    // launch) shifted to this node's centre
+    auto shift_add = [&](const auto &C, const double (&cbox)[6]) {
       double r[3];
-      for (int d = 0; d < 3; ++d) r[d] = 0.5 * (cb[side][3 + d] + cb[side][d]) - ctr[d];
+      for (int d = 0; d < 3; ++d) r[d] = 0.5 * (cbox[3 + d] + cbox[d]) - ctr[d];
       double cc[3];
       for (int i = 0; i < 3; ++i) cc[i] = C.c[i];
       for (int kk = 0; kk < 3; ++kk) {
@@ -404,11 +384,15 @@ __device__ void node_moments(BhRange rr, int2 ch, int32_t direct, const BhRange 
         for (int j = 0; j < 3; ++j) dk[j] = C.D[3 * kk + j];
         for (int i = 0; i < 3; ++i)
           for (int j = i; j < 3; ++j)
-            M.Q[6 * kk + qidx(i, j)] += C.Q[6 * kk + qidx(i, j)] + r[j] * dk[i] + r[i] * dk[j] +
+            M.Q[6 * kk + qidx(i, j)] += (double)C.Q[6 * kk + qidx(i, j)] + r[j] * dk[i] + r[i] * dk[j] +
                                         cc[kk] * (r[i] * r[j]);
         for (int j = 0; j < 3; ++j) M.D[3 * kk + j] += dk[j] + cc[kk] * r[j];
         M.c[kk] += cc[kk];
       }
+    };
+    for (int side = 0; side < 2; ++side) {
+      if (rc[side].hi - rc[side].lo <= direct) shift_add(mom[cid[side]], cb[side]);
+      else shift_add(m64[cid[side]], cb[side]);
     }
   }
 }
@@ -416,13 +400,12 @@ __device__ void node_moments(BhRange rr, int2 ch, int32_t direct, const BhRange 
 // Bottom-up moments of one level (children already done) and the node's FP32
 // records.  A node of at most `direct` segments (leaf buckets and the small
 // subtrees above them, which are most of the nodes) gets direct_records and
-// keeps no FP64 record; a larger node adds
-// such a child's segments about its own centre directly (the closed form
-// shifted exactly) and a larger child's FP64 record with the shift formulas
-// of the reference (compute_link.cpp:358-389, Eqs. 21-23).  Only nodes above
+// keeps no FP64 record; a larger node shifts each child's moments to its own
+// centre with the shift formulas of the reference (compute_link.cpp:358-389,
+// Eqs. 21-23): a larger child's FP64 record, or a small child's FP32 record.  Only nodes above
 // `direct` segments write FP64 records, so the bottom levels never round-trip
 // them through memory.  Children are read one at a time.
-__global__ __launch_bounds__(128) void bh_moments(const BhRange *__restrict__ range, int32_t n,
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) void bh_moments(const BhRange *__restrict__ range, int32_t n,
                                                   int32_t base, int32_t direct,
                                                   const int2 *__restrict__ child,
                                                   const double2 *__restrict__ sp,
@@ -445,7 +428,7 @@ __global__ __launch_bounds__(128) void bh_moments(const BhRange *__restrict__ ra
       direct_records(rr, ch, sp, gr, m);
     } else {
       fp64 = true;
-      node_moments(rr, ch, direct, range, sp, m64, M);
+      node_moments(rr, ch, direct, range, sp, m64, mom, M);
       node_records(M, ch, rr, gr, m);
     }
   }
