@@ -4,7 +4,7 @@
 // traversal) and src/linking_number_certificate.cpp:102-229 (adaptive beta).
 //
 // Trees.  All segments of the curves in Barnes–Hut pairs are sorted once by
-// (curve, 48-bit Morton code of the segment midpoint in the curve's box), a
+// (curve, 30-bit Morton code of the segment midpoint in the curve's box), a
 // single radix sort.  Each curve's tree is the balanced median split of its
 // Morton-ordered segments (left half floor(n/2)), down to leaf buckets of at
 // most leaf_size segments; a subtree is a contiguous range of the sorted order
@@ -12,9 +12,9 @@
 // by level with scan-assigned child slots (deterministic numbering).
 //
 // Moments (reference compute_link.cpp:333-389) are built bottom-up in FP64
-// about each node's box centre: leaf buckets sum their segments' closed-form
-// moments, internal nodes shift their children (Eqs. 21-23), then the records
-// are stored in FP32.  The centre keeps an FP32 hi/lo split so node
+// about each node's box centre: nodes of at most 8 segments sum their
+// segments' closed forms (FP32 terms, FP64 centring), larger nodes shift their
+// children's moments in FP64 (Eqs. 21-23), and the records are stored in FP32.  The centre keeps an FP32 hi/lo split so node
 // separations are exact to FP32 precision of the separation itself.
 //
 // Traversal.  Work items (pair, node1, node2) live in level buffers; one
@@ -1137,8 +1137,9 @@ void build_forest(flk_ctx *ctx, const std::vector<int32_t> &curves_in, int leaf,
 // level that fits one chunk is consumed whole and its buffer swapped with the
 // next level's (breadth-first, two buffers), a wider one pushes the next
 // level on a LIFO stack so memory stays bounded by depth x chunk.  The host
-// reads two queue counts per launch (pinned memory); the device counters
-// rotate through a pre-zeroed array instead of a clear per launch.
+// reads the queue counts (pinned memory) once per round of up to four levels;
+// the device counters rotate through a pre-zeroed array instead of a clear
+// per launch.
 void traverse(flk_ctx *ctx, Forest &F, const std::vector<int32_t> &ra, const std::vector<int32_t> &rb,
               const std::vector<float> &beta2, float eps, double *d_sum, double *d_err,
               unsigned long long stats[3], bool use_mlp, float log_tol, bool timing) {
