@@ -125,6 +125,24 @@ class FlkDiscInfo(C.Structure):
     ]
 
 
+class FlkTreeNode(C.Structure):
+    """One Barnes–Hut tree node in input coordinates (include/flk.h flk_tree_node)."""
+    _fields_ = [
+        ("center", C.c_double * 3),
+        ("radius", C.c_double),
+        ("rsq", C.c_double),
+        ("c", C.c_double * 3),
+        ("D", C.c_double * 9),
+        ("Q", C.c_double * 27),
+        ("cn", C.c_double),
+        ("dn", C.c_double),
+        ("qn", C.c_double),
+        ("child", C.c_int32 * 2),
+        ("seg0", C.c_int32),
+        ("nseg", C.c_int32),
+    ]
+
+
 SEG_BSPLINE, SEG_CATMULLROM, SEG_BEZIER, SEG_POLYLINE = 0, 1, 2, 3
 
 FLK_EXPORTS = [
@@ -143,6 +161,8 @@ FLK_EXPORTS = [
      [C.c_void_p, _P(C.c_int32), C.c_int64, _P(FlkParams), _P(C.c_double), _P(C.c_double),
       _P(C.c_uint32)]),
     ("flk_get_stats", C.c_int32, [C.c_void_p, _P(FlkStats)]),
+    ("flk_tree_moments", C.c_int32,
+     [C.c_void_p, C.c_int32, C.c_int32, _P(FlkTreeNode), C.c_int32, _P(C.c_int32), _P(C.c_int32)]),
     ("flk_discretize", C.c_int32,
      [C.c_void_p, _P(C.c_double), _P(C.c_int64), _P(C.c_int32), _P(C.c_uint8), C.c_int32, _P(C.c_int32),
       C.c_int64, _P(_P(C.c_double)), _P(_P(C.c_int64)), _P(FlkDiscInfo)]),
@@ -348,6 +368,26 @@ class Engine:
                                           _ptr(flags, C.c_uint32))
         self._check(rc, "flk_linking_numbers")
         return LinkResult(lk, err, flags, self.stats())
+
+    def tree_moments(self, curve: int, leaf_size: int = 1) -> tuple[dict, np.ndarray]:
+        """The Barnes–Hut tree of one uploaded curve, as flk_linking_numbers
+        builds it: per-node arrays (center, radius, rsq, c, D (n,3,3),
+        Q (n,3,3,3), cn, dn, qn, child (n,2), seg0, nseg), root first, and the
+        curve segment at each position of the tree's order."""
+        nn = C.c_int32()
+        self._check(self.lib.flk_tree_moments(self.h, curve, leaf_size, None, 0, C.byref(nn), None),
+                    "flk_tree_moments")
+        n = nn.value
+        nodes = (FlkTreeNode * max(n, 1))()
+        order = np.zeros(int(self.sizes[curve]) if n else 0, dtype=np.int32)
+        self._check(self.lib.flk_tree_moments(self.h, curve, leaf_size, nodes, n, C.byref(nn),
+                                              _ptr(order, C.c_int32) if n else None), "flk_tree_moments")
+        raw = np.ctypeslib.as_array(C.cast(nodes, C.POINTER(C.c_uint8)), shape=(C.sizeof(FlkTreeNode) * n,))
+        rec = np.frombuffer(raw.tobytes(), dtype=np.dtype(
+            [("center", "<f8", 3), ("radius", "<f8"), ("rsq", "<f8"), ("c", "<f8", 3), ("D", "<f8", (3, 3)),
+             ("Q", "<f8", (3, 3, 3)), ("cn", "<f8"), ("dn", "<f8"), ("qn", "<f8"), ("child", "<i4", 2),
+             ("seg0", "<i4"), ("nseg", "<i4")]), count=n)
+        return {k: rec[k].copy() for k in rec.dtype.names}, order
 
     def stats(self) -> dict:
         s = FlkStats()

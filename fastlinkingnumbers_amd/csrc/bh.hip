@@ -132,7 +132,7 @@ __global__ __launch_bounds__(256) void bh_split(const BhRange *__restrict__ lev,
   }
 }
 
-__device__ __forceinline__ int qidx(int i, int j) {  // symmetric 3x3 -> 6
+__host__ __device__ __forceinline__ int qidx(int i, int j) {  // symmetric 3x3 -> 6
   return i <= j ? (i == 0 ? j : (i == 1 ? 2 + j : 5)) : (j == 0 ? i : (j == 1 ? 2 + i : 5));
 }
 
@@ -1359,6 +1359,53 @@ int64_t mlp_samples(flk_ctx *ctx, int32_t a, int32_t b, int64_t cap, uint64_t se
   FLK_HIP(hipMemcpyAsync(y, dy.ptr, sizeof(float) * got, hipMemcpyDeviceToHost, s));
   FLK_HIP(hipStreamSynchronize(s));
   return got;
+}
+
+void tree_export(flk_ctx *ctx, int32_t curve, int32_t leaf, std::vector<flk_tree_node> &nodes,
+                 std::vector<int32_t> &seg_order) {
+  Curves &cv = ctx->curves;
+  hipStream_t s = ctx->stream;
+  nodes.clear();
+  seg_order.clear();
+  if (cv.n[curve] < 2) return;
+  Forest F(ctx->bh);
+  build_forest(ctx, std::vector<int32_t>{curve}, leaf, F);
+  const int32_t nn = F.nnodes, nseg = cv.n[curve];
+  std::vector<BhGeo> g(nn);
+  std::vector<BhMom> m(nn);
+  std::vector<uint32_t> ord(nseg);
+  FLK_HIP(hipMemcpyAsync(g.data(), F.geo.ptr, sizeof(BhGeo) * nn, hipMemcpyDeviceToHost, s));
+  FLK_HIP(hipMemcpyAsync(m.data(), F.mom.ptr, sizeof(BhMom) * nn, hipMemcpyDeviceToHost, s));
+  FLK_HIP(hipMemcpyAsync(ord.data(), F.order.ptr, sizeof(uint32_t) * nseg, hipMemcpyDeviceToHost, s));
+  FLK_HIP(hipStreamSynchronize(s));
+  // stored = (input - origin) * scale with a power-of-two scale: the
+  // conversions back are exact apart from the origin's addition
+  const double is = 1.0 / cv.scale, is2 = is * is, is3 = is2 * is;
+  nodes.resize(nn);
+  for (int32_t k = 0; k < nn; ++k) {
+    const BhGeo &a = g[k];
+    const BhMom &b = m[k];
+    flk_tree_node &o = nodes[k];
+    std::memset(&o, 0, sizeof o);
+    const double ctr[3] = {(double)a.cx + (double)a.lx, (double)a.cy + (double)a.ly, (double)a.cz + (double)a.lz};
+    for (int d = 0; d < 3; ++d) o.center[d] = cv.origin[d] + ctr[d] * is;
+    o.radius = (double)a.radius * is;
+    o.rsq = (double)a.rsq * is2;
+    for (int d = 0; d < 3; ++d) o.c[d] = (double)b.c[d] * is;
+    for (int t = 0; t < 9; ++t) o.D[t] = (double)b.D[t] * is2;
+    for (int kk = 0; kk < 3; ++kk)
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) o.Q[9 * kk + 3 * i + j] = (double)b.Q[6 * kk + qidx(i, j)] * is3;
+    o.cn = (double)a.cn * is;
+    o.dn = (double)a.dn * is2;
+    o.qn = (double)a.qn * is3;
+    o.child[0] = a.c0 < 0 ? -1 : a.c0;
+    o.child[1] = a.c0 < 0 ? -1 : a.c1;
+    o.seg0 = a.seg0;
+    o.nseg = a.nseg;
+  }
+  seg_order.resize(nseg);
+  for (int32_t t = 0; t < nseg; ++t) seg_order[t] = (int32_t)((int64_t)ord[t] - cv.off[curve]);
 }
 
 namespace {

@@ -346,6 +346,26 @@ int32_t flk_get_stats(const flk_ctx *ctx, flk_stats *out) {
   return FLK_OK;
 }
 
+int32_t flk_tree_moments(flk_ctx *ctx, int32_t curve, int32_t leaf_size, flk_tree_node *nodes,
+                         int32_t cap, int32_t *nnodes, int32_t *seg_order) {
+  if (!ctx || !nnodes) return FLK_ERR_ARG;
+  FLK_GUARD(ctx, {
+    FLK_HIP(hipSetDevice(ctx->device));
+    if (curve < 0 || curve >= ctx->curves.ncurves) throw Error(FLK_ERR_ARG, "curve index out of range");
+    if (leaf_size < 1) throw Error(FLK_ERR_ARG, "leaf_size must be >= 1");
+    std::vector<flk_tree_node> t;
+    std::vector<int32_t> order;
+    flk::tree_export(ctx, curve, leaf_size, t, order);
+    *nnodes = (int32_t)t.size();
+    if (nodes) {
+      if (cap < (int32_t)t.size()) throw Error(FLK_ERR_ARG, "cap is smaller than the node count");
+      std::copy(t.begin(), t.end(), nodes);
+    }
+    if (seg_order) std::copy(order.begin(), order.end(), seg_order);
+    return FLK_OK;
+  })
+}
+
 int32_t flk_discretize(flk_ctx *ctx, const double *ctrl_xyz, const int64_t *offsets,
                        const int32_t *segment_types, const uint8_t *looped, int32_t ncurves,
                        const int32_t *pairs, int64_t npairs, const double **out_xyz,

@@ -265,6 +265,11 @@ void discretize_run(flk_ctx *ctx, const double *ctrl, const int64_t *offsets, co
 void bh_run(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, const flk_params &prm,
             std::vector<double> &lk, std::vector<double> &err, std::vector<uint32_t> &flags);
 
+// The Barnes–Hut tree of one curve (as bh_run builds it), copied to the host
+// in input coordinates, and the curve segment at each position of its order.
+void tree_export(flk_ctx *ctx, int32_t curve, int32_t leaf, std::vector<flk_tree_node> &nodes,
+                 std::vector<int32_t> &seg_order);
+
 void pls_run(flk_ctx *ctx, const double *boxes, int32_t ncurves, std::vector<int32_t> &out);
 
 void curve_boxes_run(flk_ctx *ctx, std::vector<double> &boxes);

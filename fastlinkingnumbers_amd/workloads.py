@@ -125,12 +125,14 @@ def random_walk_pair(seed: int = 0, n: int = 2_000_000, rg: float = 100.0, workl
 
 # ---------------------------------------------------------------- workload 4
 def helical_lattice(seed: int = 0, nloops: int = 10000, per_turn: int = 101, nmin: int = 64,
-                    nmax: int = 262144, pitch: float = 1.5, amp: float = 0.1) -> list[np.ndarray]:
+                    nmax: int = 262144, pitch: float = 1.6, amp: float = 0.1) -> list[np.ndarray]:
     """The chain-mail sheet wrapped into a tube: ring k sits at angle
     2 pi k / per_turn on a cylinder whose circumference is per_turn * pitch,
     rising pitch per turn, so ring k neighbours k +- 1 along the helix and
     k +- per_turn across turns (per_turn odd keeps the tilt alternating both
-    ways).  Each ring carries 3 random Fourier modes (total amplitude <= amp)."""
+    ways).  Each ring carries 3 random Fourier modes (total amplitude <= amp).
+    At pitch 1.6 a ring links 0-4 of its neighbours (mostly 2-3, LK +-1): the
+    Fourier modes unlink some neighbour pairs that pitch 1.5 keeps linked."""
     g = _rng(4, seed)
     ns = _loglog_counts(g, nloops, nmin, nmax)
     Rt = per_turn * pitch / (2 * np.pi)

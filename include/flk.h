@@ -145,6 +145,35 @@ int32_t flk_linking_numbers(flk_ctx *ctx, const int32_t *pairs, int64_t npairs,
 /* Statistics of the last flk_linking_numbers call. */
 int32_t flk_get_stats(const flk_ctx *ctx, flk_stats *out);
 
+/* One node of a curve's Barnes–Hut tree, in input coordinates.  Moments are
+ * about the node's box centre, with the reference's index convention
+ * (src/compute_link.h:95-114): D[3k+j] = int r'_k (r - center)_j and
+ * Q[9k+3i+j] = int r'_k (r - center)_i (r - center)_j; the norms feed the
+ * Eq. 14 error estimate (src/compute_link.cpp:383-388). */
+typedef struct flk_tree_node {
+  double center[3];   /* box centre (FP32 hi + lo, as the traversal uses it) */
+  double radius;      /* half box diagonal */
+  double rsq;         /* radius squared as the traversal compares it */
+  double c[3];        /* monopole, int r' */
+  double D[9];
+  double Q[27];
+  double cn, dn, qn;  /* |c|, |D|_F / sqrt 3, sqrt(sum_k |Q_k|_F^2) / 3 */
+  int32_t child[2];   /* node indices; -1 for a leaf bucket */
+  int32_t seg0, nseg; /* range of the tree's segment order */
+} flk_tree_node;
+
+/* Builds the Barnes–Hut tree of uploaded curve `curve` with leaf buckets of at
+ * most leaf_size segments -- the same trees and FP32 records the Barnes–Hut
+ * path of flk_linking_numbers builds -- and copies it out: *nnodes nodes,
+ * root first (nodes may be NULL to query the count; otherwise cap must hold
+ * them), and seg_order[s] (may be NULL; n entries for a curve of n segments)
+ * = the curve segment at position s of the tree's order.  A curve of fewer
+ * than 2 vertices has no tree (*nnodes = 0).  Replaces MakeSegmentTrees and
+ * BuildTreeMoments (src/compute_link.h:128-151, src/compute_link.cpp:294-389)
+ * for callers that inspect the trees. */
+int32_t flk_tree_moments(flk_ctx *ctx, int32_t curve, int32_t leaf_size, flk_tree_node *nodes,
+                         int32_t cap, int32_t *nnodes, int32_t *seg_order);
+
 /* Segment types of flk_discretize, in the order of the reference's
  * SegmentType (include/curve.h). */
 #define FLK_SEG_BSPLINE 0

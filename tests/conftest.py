@@ -18,5 +18,17 @@ def pytest_configure(config):
 def engine():
     import fastlinkingnumbers_amd as flk
     eng = flk.Engine(0)
+    eng.default_tune = eng.get_tune()
     yield eng
     eng.close()
+
+
+@pytest.fixture(autouse=True)
+def _restore_tune(request):
+    """A test that changes the shared engine's launch shapes does not leak
+    them into later tests: the default shapes are restored after each one."""
+    yield
+    if "engine" in request.fixturenames:
+        eng = request.getfixturevalue("engine")
+        if getattr(eng, "h", None):
+            eng.set_tune(**eng.default_tune)

@@ -1,8 +1,10 @@
 """GPU parity of the direct-sum tile kernels (FP32 and FP64) against the oracle.
 
 Tolerances: FP64 kernel vs the FP64 oracle within 1e-9 per pair; FP32 kernel
-within 2e-3 per pair at these sizes (and certificates identical).  The
-reference's own answer for the fixtures is pinned through its certificates.
+within 1e-5 per pair at these sizes (measured errors are below 3e-7, so a
+biased angle polynomial or a wrong polarization term fails), and certificates
+identical.  The reference's own answer for the fixtures is pinned through its
+certificates.
 """
 import os
 
@@ -31,7 +33,7 @@ def test_config1_all_pairs_fp32_fp64(engine):
     r64 = engine.linking_numbers(pairs, method="directsum", fp64=1)
     print("fp32 max err", np.abs(r32.lk - ref).max(), "fp64 max err", np.abs(r64.lk - ref).max())
     assert np.abs(r64.lk - ref).max() < 1e-9
-    assert np.abs(r32.lk - ref).max() < 2e-3
+    assert np.abs(r32.lk - ref).max() < 1e-5
     cert = flk.certificate_from_lk(pairs, r32.lk)
     assert cert == flk.certificate_from_lk(pairs, ref) == [(0, 1, -1), (2, 3, -2)]
     assert not np.any(r32.flags & flk.FLK_PAIR_FALLBACK)
@@ -73,7 +75,7 @@ def test_uneven_batch_matches_oracle(engine):
     res = engine.linking_numbers(pairs, method="directsum")
     ref = np.array([O.ds(curves[i], curves[j]) for i, j in pairs])
     print("pairs", len(pairs), "max err", np.abs(res.lk - ref).max())
-    assert np.abs(res.lk - ref).max() < 2e-3
+    assert np.abs(res.lk - ref).max() < 1e-5
     assert flk.certificate_from_lk(pairs, res.lk) == flk.certificate_from_lk(pairs, ref)
 
 

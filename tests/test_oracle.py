@@ -251,6 +251,22 @@ def test_far_field_equals_taylor_expansion_and_converges():
         prev = e
 
 
+def test_host_traversal_restatement_reproduces_oracle_bh():
+    """tests/bh_host.py (which evaluates the device's exported trees on the
+    GPU side) run over the oracle's own kd trees equals the oracle's
+    Barnes–Hut: same visits, same value and estimate to FP64 summation
+    order (frontiers stay below the reference's 75,000 BFS limit here)."""
+    import bh_host as H
+    from fastlinkingnumbers_amd import workloads as W
+    for n, rg, beta in ((800, 6.0, 2.0), (600, 8.0, 1.5)):
+        c, d = W.random_walk_pair(seed=4, n=n, rg=rg, workload=3)
+        r = H.traverse(H.from_oracle(c), H.from_oracle(d), beta)
+        v, est, vis = O.bh_pair(c, d, beta=beta)
+        assert r["visits"] == vis
+        assert abs(r["lk"] - v) < 1e-11 and abs(r["est"] - est) < 1e-11 * max(1.0, est)
+        assert r["far"] > 0 and r["leaf_items"] > 0
+
+
 def test_pls_tree_equals_brute_force_and_touching():
     g = np.random.default_rng(5)
     curves = []
