@@ -73,7 +73,9 @@ class FlkStats(C.Structure):
         ("ds_waves", C.c_int32),
         ("ms_bh_level", C.c_double),
         ("ms_mlp", C.c_double),
-        ("reserved", C.c_int64 * 6),
+        ("mlp_evals", C.c_int64),
+        ("bh_host_passes", C.c_int64),
+        ("reserved", C.c_int64 * 4),
     ]
 
     def as_dict(self) -> dict:
@@ -102,7 +104,8 @@ class FlkTune(C.Structure):
         ("ds_tasks_per_wave", C.c_int32),
         ("bh_items_per_thread", C.c_int32),
         ("ingest_threads", C.c_int32),
-        ("reserved", C.c_int32 * 3),
+        ("bh_level_capacity", C.c_int32),
+        ("reserved", C.c_int32 * 2),
     ]
 
     def as_dict(self) -> dict:

@@ -670,14 +670,20 @@ __global__ __launch_bounds__(kTravBlock) void bh_level(
     double *__restrict__ psum, double *__restrict__ perr) {
   constexpr int kTravItems = kTravBlock * IPT;
   // A level enqueued before its count reached the host reads the count from
-  // the previous level's queue counter; its grid covers the bound n.
-  if (n_dev) n = min(n, (int64_t)*n_dev);
-  if ((int64_t)blockIdx.x * kTravItems >= n) return;  // whole block: before any barrier
+  // the previous level's queue counter; n is then the bound (the buffer's
+  // capacity).  A block takes chunks blockIdx.x, blockIdx.x + gridDim.x, ...:
+  // one chunk per block when the grid covers the bound, several on a
+  // persistent grid.  The loop condition is uniform across the block.  A
+  // count above the bound means the previous level overflowed its buffer
+  // (its last item may be unwritten): the level is skipped, and the host,
+  // seeing the count, discards the pass.
+  if (n_dev) n = level_count(n_dev, n);
   __shared__ int4 stage[kTravItems];
   __shared__ TravShared sh;
+  for (int64_t chunk = blockIdx.x; chunk * kTravItems < n; chunk += gridDim.x) {
   if (threadIdx.x == 0) sh.nsplit = sh.nleaf = 0;
   __syncthreads();
-  const int64_t b0 = (int64_t)blockIdx.x * kTravItems + threadIdx.x;
+  const int64_t b0 = chunk * kTravItems + threadIdx.x;
   int tp = -1;  // this thread's far-field pair and its running sums
   double tv = 0.0, te = 0.0;
 #pragma unroll 1
@@ -757,23 +763,29 @@ __global__ __launch_bounds__(kTravBlock) void bh_level(
       leaves[s] = BhItem{r.x, r.y, r.z, 0};
     }
   }
+  __syncthreads();  // the stage and the shared counters are reused by the next chunk
+  }
 }
 
 // Exact evaluation of leaf-bucket pairs: every segment pair of the two
 // buckets with the reference leaf term, from FP32 hi/lo vertices.  Blocks of
-// 256 * IPT items as in bh_level; counters[2] gets the segment-pair count.
+// 256 * IPT items as in bh_level (chunks strided over the grid; with n_dev
+// the count is read on the device, n being the bound); counters[2] gets the
+// segment-pair count.
 template <int IPT>
 __global__ __launch_bounds__(kTravBlock) void bh_leaf_pairs(
-    const BhItem *__restrict__ items, int64_t n, const BhGeo *__restrict__ geo,
-    const uint32_t *__restrict__ order, const float4 *__restrict__ hi,
+    const BhItem *__restrict__ items, int64_t n, const unsigned long long *__restrict__ n_dev,
+    const BhGeo *__restrict__ geo, const uint32_t *__restrict__ order, const float4 *__restrict__ hi,
     const float4 *__restrict__ lo, const float4 *__restrict__ ed, double *__restrict__ psum,
     double *__restrict__ perr, unsigned long long *__restrict__ counters) {
   constexpr int kTravItems = kTravBlock * IPT;
+  if (n_dev) n = level_count(n_dev, n);
   __shared__ TravShared sh;
-  const int64_t b0 = (int64_t)blockIdx.x * kTravItems + threadIdx.x;
+  int npairs = 0;
+  for (int64_t chunk = blockIdx.x; chunk * kTravItems < n; chunk += gridDim.x) {
+  const int64_t b0 = chunk * kTravItems + threadIdx.x;
   int tp = -1;
   double tv = 0.0;
-  int npairs = 0;
 #pragma unroll 1
   for (int k = 0; k < IPT; ++k) {
     const int64_t t = b0 + (int64_t)k * kTravBlock;
@@ -803,6 +815,8 @@ __global__ __launch_bounds__(kTravBlock) void bh_leaf_pairs(
     }
   }
   block_pair_add(tp, tp >= 0, tv, 0.0, psum, perr, sh);
+  __syncthreads();  // sh is reused by the next chunk
+  }
   int tot = npairs;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off, 64);
@@ -930,6 +944,14 @@ __global__ void bh_roots(const int32_t *__restrict__ ra, const int32_t *__restri
   if (k < n) out[k] = BhItem{(int32_t)k, ra[k], rb[k], 0};
 }
 
+// Clears the counters of a device-scheduled pass and sets the root level's
+// count (a kernel rather than a memset, so the pass can be captured).
+__global__ void bh_pass_init(unsigned long long *__restrict__ cnt, int n, int root_slot,
+                             unsigned long long roots) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) cnt[k] = k == root_slot ? roots : 0ull;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- host side
@@ -942,6 +964,7 @@ constexpr int32_t kDirectSegments = 8;
 
 struct Forest {
   int32_t nnodes = 0;
+  int32_t depth = 0;          // node levels of the deepest tree
   std::vector<int32_t> root;  // per curve (-1 if no tree)
   DevBuf &geo, &mom, &order;
   explicit Forest(BhScratch &b) : geo(b.geo), mom(b.mom), order(b.order) {}
@@ -1091,6 +1114,7 @@ void build_forest(flk_ctx *ctx, const std::vector<int32_t> &curves_in, int leaf,
     total += 2 * nsplit;
   }
   F.nnodes = total;
+  F.depth = (int32_t)level_base.size();
   // FP64 records exist only for nodes above `direct` segments: node ids are
   // assigned level by level, so they all fall below the first level whose
   // ranges are all that small
@@ -1142,7 +1166,7 @@ void build_forest(flk_ctx *ctx, const std::vector<int32_t> &curves_in, int leaf,
 // per launch.
 void traverse(flk_ctx *ctx, Forest &F, const std::vector<int32_t> &ra, const std::vector<int32_t> &rb,
               const std::vector<float> &beta2, float eps, double *d_sum, double *d_err,
-              unsigned long long stats[3], bool use_mlp, float log_tol, bool timing) {
+              unsigned long long stats[4], bool use_mlp, float log_tol, bool timing) {
   hipStream_t s = ctx->stream;
   BhScratch &B = ctx->bh;
   const int64_t np = (int64_t)ra.size();
@@ -1162,7 +1186,8 @@ void traverse(flk_ctx *ctx, Forest &F, const std::vector<int32_t> &ra, const std
   FLK_HIP(hipMemcpyAsync(B.roots_a.ptr, ra.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice, s));
   FLK_HIP(hipMemcpyAsync(B.roots_b.ptr, rb.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice, s));
   FLK_HIP(hipMemcpyAsync(B.beta2.ptr, beta2.data(), sizeof(float) * np, hipMemcpyHostToDevice, s));
-  unsigned long long *cnt = B.cnt.as<unsigned long long>();  // [2] leaf segment pairs, [4 + 2k] queues
+  // [2] leaf segment pairs, [3] network evaluations, [4 + 2k] queues
+  unsigned long long *cnt = B.cnt.as<unsigned long long>();
   FLK_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * (4 + 2 * kSlots), s));
 
   struct Level {
@@ -1233,7 +1258,7 @@ void traverse(flk_ctx *ctx, Forest &F, const std::vector<int32_t> &ra, const std
       const int64_t bound = take << d;
       const unsigned long long *n_dev = d == 0 ? nullptr : qc + 2 * (d - 1);
       if (use_mlp)
-        mlp_predict_items(ctx, in, bound, n_dev, F.geo.as<BhGeo>(), eps, B.pred.as<float>(),
+        mlp_predict_items(ctx, in, bound, n_dev, F.geo.as<BhGeo>(), eps, B.pred.as<float>(), cnt + 3,
                           timing ? ctx->ev0 : nullptr, timing ? ctx->ev3 : nullptr);
       // with timing, start/stop events ride on the dispatch itself
       const dim3 grid((unsigned)((bound + ipt * kTravBlock - 1) / (ipt * kTravBlock)));
@@ -1269,7 +1294,7 @@ void traverse(flk_ctx *ctx, Forest &F, const std::vector<int32_t> &ra, const std
       const Curves &cv = ctx->curves;
       auto leafk = ipt == 2 ? bh_leaf_pairs<2> : ipt == 8 ? bh_leaf_pairs<8> : bh_leaf_pairs<4>;
       hipLaunchKernelGGL(leafk, dim3((unsigned)((nleaf + ipt * kTravBlock - 1) / (ipt * kTravBlock))),
-                         dim3(kTravBlock), 0, s, B.leaves.as<BhItem>(), (int64_t)nleaf, F.geo.as<BhGeo>(),
+                         dim3(kTravBlock), 0, s, B.leaves.as<BhItem>(), (int64_t)nleaf, nullptr, F.geo.as<BhGeo>(),
                          F.order.as<uint32_t>(), cv.hi.as<float4>(), cv.lo.as<float4>(),
                          cv.ed.as<float4>(), d_sum, d_err, cnt);
       FLK_HIP(hipGetLastError());
@@ -1288,12 +1313,139 @@ void traverse(flk_ctx *ctx, Forest &F, const std::vector<int32_t> &ra, const std
       spare = level_buf(2 * kChunk);
     }
   }
-  unsigned long long st[3];
+  unsigned long long st[4];
   FLK_HIP(hipMemcpyAsync(st, cnt, sizeof st, hipMemcpyDeviceToHost, s));
   FLK_HIP(hipStreamSynchronize(s));
   stats[0] += visits;
   stats[1] += far;
   stats[2] += st[2];
+  stats[3] += st[3];  // learned-criterion network evaluations
+}
+
+// Blocks of `kernel` resident on the whole chip at once (the persistent grid
+// of a device-scheduled pass).
+int resident_grid(flk_ctx *ctx, const void *kernel) {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, kTravBlock, 0) != hipSuccess || nb < 1) nb = 1;
+  return nb * std::max(1, ctx->num_cus);
+}
+
+// One traversal pass with every level enqueued up front: level d reads its
+// item count from the counter that level d - 1 incremented, on a persistent
+// grid whose blocks stride over the level's chunks, so no count returns to
+// the host until the pass has ended (one read-back per pass).  Every split
+// deepens one of the two trees, so a pass has fewer than 2 * depth + 1
+// levels; launched levels past the last nonempty one see a count of 0 and
+// exit at once.  Both level buffers and the leaf list hold `cap` items.  A
+// level or leaf list beyond that, or items left after the last launched
+// level, make the pass return false: its sums are then incomplete, and the
+// caller clears them and re-runs the pairs on the host-driven schedule.
+bool traverse_device(flk_ctx *ctx, Forest &F, const std::vector<int32_t> &ra, const std::vector<int32_t> &rb,
+                     const std::vector<float> &beta2, float eps, double *d_sum, double *d_err,
+                     unsigned long long stats[4], bool use_mlp, float log_tol) {
+  hipStream_t s = ctx->stream;
+  BhScratch &B = ctx->bh;
+  const int64_t np = (int64_t)ra.size();
+  if (np == 0) return true;
+  constexpr int kMaxLevels = 1000;  // counter slots: 4 + 2 * (levels + 1) <= 4 + 2 * 1024
+  const int D = 2 * F.depth + 1;
+  if (D > kMaxLevels) return false;
+  if (B.dev_cap == 0) {
+    // two level buffers and the leaf list (16 B per item) and the predictions
+    // (4 B), within a tenth of the free memory, at most 2^26 items each
+    size_t free_b = 0, total_b = 0;
+    FLK_HIP(hipMemGetInfo(&free_b, &total_b));
+    int64_t cap = int64_t(1) << 26;
+    while (cap > (int64_t(1) << 20) && (double)cap * 52.0 > 0.1 * (double)free_b) cap >>= 1;
+    B.dev_cap = cap;
+  }
+  const int64_t cap = ctx->tune.bh_level_capacity > 0 ? (int64_t)ctx->tune.bh_level_capacity : B.dev_cap;
+  if (np > cap) return false;
+  const int ipt = (ctx->tune.bh_items_per_thread == 2 || ctx->tune.bh_items_per_thread == 8)
+                      ? ctx->tune.bh_items_per_thread
+                      : 4;
+  while (B.levels.size() < 2) B.levels.push_back(std::make_unique<DevBuf>());
+  B.levels[0]->reserve(sizeof(BhItem) * cap);
+  B.levels[1]->reserve(sizeof(BhItem) * cap);
+  B.leaves.reserve(sizeof(BhItem) * cap);
+  if (use_mlp) {
+    mlp_upload(ctx);
+    B.pred.reserve(sizeof(float) * cap);
+  }
+  B.roots_a.reserve(sizeof(int32_t) * np);
+  B.roots_b.reserve(sizeof(int32_t) * np);
+  B.beta2.reserve(sizeof(float) * np);
+  const int nslots = 4 + 2 * (D + 1);
+  B.cnt.reserve(sizeof(unsigned long long) * std::max(nslots, 4 + 2 * 1024));
+  if (B.dev_counts.size() < (size_t)nslots) B.dev_counts.resize(nslots);
+  FLK_HIP(hipMemcpyAsync(B.roots_a.ptr, ra.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice, s));
+  FLK_HIP(hipMemcpyAsync(B.roots_b.ptr, rb.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice, s));
+  FLK_HIP(hipMemcpyAsync(B.beta2.ptr, beta2.data(), sizeof(float) * np, hipMemcpyHostToDevice, s));
+  // [2] leaf segment pairs, [3] network evaluations, [4 + 2d] items of level
+  // d, [5] leaf items
+  unsigned long long *cnt = B.cnt.as<unsigned long long>();
+  hipLaunchKernelGGL(bh_pass_init, dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, s, cnt, nslots, 4,
+                     (unsigned long long)np);
+  BhItem *buf[2] = {B.levels[0]->as<BhItem>(), B.levels[1]->as<BhItem>()};
+  hipLaunchKernelGGL(bh_roots, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, B.roots_a.as<int32_t>(),
+                     B.roots_b.as<int32_t>(), np, buf[0]);
+  FLK_HIP(hipGetLastError());
+  auto lev = ipt == 2 ? bh_level<2> : ipt == 8 ? bh_level<8> : bh_level<4>;
+  auto leafk = ipt == 2 ? bh_leaf_pairs<2> : ipt == 8 ? bh_leaf_pairs<8> : bh_leaf_pairs<4>;
+  const int64_t items_per_block = (int64_t)ipt * kTravBlock;
+  const unsigned grid_lev = (unsigned)std::min<int64_t>(resident_grid(ctx, (const void *)lev),
+                                                        (cap + items_per_block - 1) / items_per_block);
+  const unsigned grid_leaf = (unsigned)std::min<int64_t>(resident_grid(ctx, (const void *)leafk),
+                                                         (cap + items_per_block - 1) / items_per_block);
+  for (int d = 0; d < D; ++d) {
+    const BhItem *in = buf[d & 1];
+    const unsigned long long *n_dev = cnt + 4 + 2 * d;
+    if (use_mlp)
+      mlp_predict_items(ctx, in, cap, n_dev, F.geo.as<BhGeo>(), eps, B.pred.as<float>(), cnt + 3, nullptr,
+                        nullptr);
+    hipLaunchKernelGGL(lev, dim3(grid_lev), dim3(kTravBlock), 0, s, in, cap, n_dev, F.geo.as<BhGeo>(),
+                       F.mom.as<BhMom>(), (const float *)B.beta2.as<float>(), eps,
+                       (const float *)(use_mlp ? B.pred.as<float>() : nullptr), log_tol, buf[(d + 1) & 1],
+                       cnt + 4 + 2 * (d + 1), cap, B.leaves.as<BhItem>(), cnt + 5, cap, d_sum, d_err);
+    FLK_HIP(hipGetLastError());
+  }
+  const Curves &cv = ctx->curves;
+  hipLaunchKernelGGL(leafk, dim3(grid_leaf), dim3(kTravBlock), 0, s, B.leaves.as<BhItem>(), cap,
+                     (const unsigned long long *)(cnt + 5), F.geo.as<BhGeo>(), F.order.as<uint32_t>(),
+                     cv.hi.as<float4>(), cv.lo.as<float4>(), cv.ed.as<float4>(), d_sum, d_err, cnt);
+  FLK_HIP(hipGetLastError());
+  unsigned long long *h = B.dev_counts.data();
+  FLK_HIP(hipMemcpyAsync(h, cnt, sizeof(unsigned long long) * nslots, hipMemcpyDeviceToHost, s));
+  FLK_HIP(hipStreamSynchronize(s));
+  // every item is exactly one of: leaf pair, far field, split (two children)
+  const unsigned long long nleaf = h[5];
+  if ((int64_t)nleaf > cap || h[4 + 2 * D] != 0) return false;
+  unsigned long long visits = 0, split_children = 0;
+  for (int d = 0; d < D; ++d) {
+    if ((int64_t)h[4 + 2 * d] > cap || (int64_t)h[4 + 2 * (d + 1)] > cap) return false;
+    visits += h[4 + 2 * d];
+    split_children += h[4 + 2 * (d + 1)];
+  }
+  stats[0] += visits;
+  stats[1] += visits - split_children / 2 - nleaf;
+  stats[2] += h[2];
+  stats[3] += h[3];
+  return true;
+}
+
+// A pass on the device schedule, or on the host-driven chunked one when
+// per-launch timing is asked for or the device schedule's buffers overflow.
+void traverse_pass(flk_ctx *ctx, Forest &F, const std::vector<int32_t> &ra, const std::vector<int32_t> &rb,
+                   const std::vector<float> &beta2, float eps, double *d_sum, double *d_err,
+                   unsigned long long stats[4], bool use_mlp, float log_tol, bool timing) {
+  if (!timing && traverse_device(ctx, F, ra, rb, beta2, eps, d_sum, d_err, stats, use_mlp, log_tol)) return;
+  if (!timing) {  // the incomplete pass's sums are discarded
+    const size_t np = ra.size();
+    FLK_HIP(hipMemsetAsync(d_sum, 0, sizeof(double) * np, ctx->stream));
+    FLK_HIP(hipMemsetAsync(d_err, 0, sizeof(double) * np, ctx->stream));
+  }
+  ctx->stats.bh_host_passes += 1;
+  traverse(ctx, F, ra, rb, beta2, eps, d_sum, d_err, stats, use_mlp, log_tol, timing);
 }
 
 }  // namespace
@@ -1443,9 +1595,9 @@ void bh_batch(flk_ctx *ctx, const int32_t *pairs, const std::vector<int64_t> &bh
   double *d_sum = ctx->bh.sum.as<double>(), *d_err = ctx->bh.err.as<double>();
   FLK_HIP(hipMemsetAsync(d_sum, 0, sizeof(double) * 2 * nb, s));
   FLK_HIP(hipMemsetAsync(d_err, 0, sizeof(double) * 2 * nb, s));
-  unsigned long long st[3] = {0, 0, 0};
+  unsigned long long st[4] = {0, 0, 0, 0};
   t0 = std::chrono::steady_clock::now();
-  traverse(ctx, F, ra, rb, beta2, eps, d_sum, d_err, st, use_mlp, (float)prm.mlp_log_tol, prm.timing != 0);
+  traverse_pass(ctx, F, ra, rb, beta2, eps, d_sum, d_err, st, use_mlp, (float)prm.mlp_log_tol, prm.timing != 0);
   std::vector<double> v(nb), e(nb);
   FLK_HIP(hipMemcpyAsync(v.data(), d_sum, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
   FLK_HIP(hipMemcpyAsync(e.data(), d_err, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
@@ -1482,7 +1634,7 @@ void bh_batch(flk_ctx *ctx, const int32_t *pairs, const std::vector<int64_t> &bh
   if (!redo.empty()) {
     const int64_t nr = (int64_t)redo.size();
     double *d_sum2 = d_sum + nb, *d_err2 = d_err + nb;
-    traverse(ctx, F, ra2, rb2, b2, eps, d_sum2, d_err2, st, use_mlp, (float)prm.mlp_log_tol - 1.0f,
+    traverse_pass(ctx, F, ra2, rb2, b2, eps, d_sum2, d_err2, st, use_mlp, (float)prm.mlp_log_tol - 1.0f,
              prm.timing != 0);
     std::vector<double> v2(nr), e2(nr);
     FLK_HIP(hipMemcpyAsync(v2.data(), d_sum2, sizeof(double) * nr, hipMemcpyDeviceToHost, s));
@@ -1500,6 +1652,7 @@ void bh_batch(flk_ctx *ctx, const int32_t *pairs, const std::vector<int64_t> &bh
   ctx->stats.bh_visits += (int64_t)st[0];
   ctx->stats.bh_far += (int64_t)st[1];
   ctx->stats.bh_leaf_pairs += (int64_t)st[2];
+  ctx->stats.mlp_evals += (int64_t)st[3];
   for (int64_t t = 0; t < nb; ++t) {
     lk[bh_idx[t]] = v[t];
     err[bh_idx[t]] = e[t];

@@ -23,6 +23,15 @@ struct BhItem {
   int32_t pair, a, b, pad;
 };
 
+// Items of a level whose count was produced on the device, for a kernel whose
+// buffer holds `bound` items: a count above the bound means the producer
+// overflowed the buffer (its last slot may be unwritten), and the level is
+// skipped (0 items); the host sees the count and discards the pass.
+__device__ __forceinline__ int64_t level_count(const unsigned long long *n_dev, int64_t bound) {
+  const unsigned long long c = *n_dev;
+  return c <= (unsigned long long)bound ? (int64_t)c : 0;
+}
+
 // Natural log of a positive normal float (every feature argument is at least
 // 1e-12 or a segment count): the hardware log2 times ln 2, without the
 // denormal range reduction of the general logf.
@@ -33,24 +42,30 @@ __device__ __forceinline__ float ln_normal(float x) { return __builtin_amdgcn_lo
 // d^-3, the segment counts and the Eq. 14 error estimate of the reference
 // (compute_link.cpp:517-522) in linking-number units (all natural logs,
 // floored at log 1e-12).  Scale-free, so a network trained on one model size
-// applies to another.
-__device__ __forceinline__ void mlp_features(const BhGeo &A, const BhGeo &B, float dsq, float *f) {
+// applies to another.  kLog2: base-2 logs instead (the inference kernel folds
+// the factor ln 2 into its input normalisation).
+template <bool kLog2>
+__device__ __forceinline__ void mlp_features_t(const BhGeo &A, const BhGeo &B, float dsq, float *f) {
+  auto lg = [](float x) { return kLog2 ? __builtin_amdgcn_logf(x) : ln_normal(x); };
   const float id = __builtin_amdgcn_rsqf(dsq), id2 = id * id;
   const float tiny = 1e-12f;
-  f[0] = ln_normal(A.radius * id + tiny);
-  f[1] = ln_normal(B.radius * id + tiny);
-  f[2] = ln_normal((A.radius + B.radius) * id + tiny);
-  f[3] = ln_normal(A.cn * id + tiny);
-  f[4] = ln_normal(A.dn * id2 + tiny);
-  f[5] = ln_normal(A.qn * id2 * id + tiny);
-  f[6] = ln_normal(B.cn * id + tiny);
-  f[7] = ln_normal(B.dn * id2 + tiny);
-  f[8] = ln_normal(B.qn * id2 * id + tiny);
-  f[9] = ln_normal((float)A.nseg);
-  f[10] = ln_normal((float)B.nseg);
+  f[0] = lg(A.radius * id + tiny);
+  f[1] = lg(B.radius * id + tiny);
+  f[2] = lg((A.radius + B.radius) * id + tiny);
+  f[3] = lg(A.cn * id + tiny);
+  f[4] = lg(A.dn * id2 + tiny);
+  f[5] = lg(A.qn * id2 * id + tiny);
+  f[6] = lg(B.cn * id + tiny);
+  f[7] = lg(B.dn * id2 + tiny);
+  f[8] = lg(B.qn * id2 * id + tiny);
+  f[9] = lg((float)A.nseg);
+  f[10] = lg((float)B.nseg);
   const float est = 0.0795774715f * id2 * id2 * id *
                     (A.qn * (A.radius * B.cn + 3.0f * B.dn) + B.qn * (B.radius * A.cn + 3.0f * A.dn));
-  f[11] = ln_normal(est + tiny);
+  f[11] = lg(est + tiny);
+}
+__device__ __forceinline__ void mlp_features(const BhGeo &A, const BhGeo &B, float dsq, float *f) {
+  mlp_features_t<false>(A, B, dsq, f);
 }
 
 }  // namespace flk

@@ -129,6 +129,47 @@ void curves_finish(flk_ctx *ctx) {
 
 }  // namespace flk
 
+namespace {
+
+// Fallback and judge: the FP64 direct sum replaces every pair that lands
+// farther than fallback_tol from an integer (or is not finite); then the NaN
+// flags and the caller's outputs.
+void judge_and_finish(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, const flk_params &prm,
+                      std::vector<double> &v, const std::vector<double> &e, std::vector<uint32_t> &f, double *lk,
+                      double *err_est, uint32_t *flags) {
+  if (prm.fallback_tol > 0 && !(prm.method == FLK_DIRECTSUM && prm.fp64)) {
+    std::vector<int32_t> redo;
+    std::vector<int64_t> idx;
+    for (int64_t k = 0; k < npairs; ++k) {
+      const double d = std::fabs(v[k] - std::round(v[k]));
+      if (!(d <= prm.fallback_tol)) {
+        redo.push_back(pairs[2 * k]);
+        redo.push_back(pairs[2 * k + 1]);
+        idx.push_back(k);
+      }
+    }
+    if (!idx.empty()) {
+      std::vector<double> w;
+      float ms = 0.f;
+      flk::ds_run(ctx, redo.data(), (int64_t)idx.size(), true, w, &ms, prm.graph != 0);
+      ctx->stats.ms_fallback = ms;
+      ctx->stats.fallbacks = (int64_t)idx.size();
+      for (size_t t = 0; t < idx.size(); ++t) {
+        v[idx[t]] = w[t];
+        f[idx[t]] |= FLK_PAIR_FALLBACK;
+      }
+    }
+  }
+  for (int64_t k = 0; k < npairs; ++k) {
+    if (std::isnan(v[k])) f[k] |= FLK_PAIR_NAN;
+    lk[k] = v[k];
+    if (err_est) err_est[k] = e[k];
+    if (flags) flags[k] = f[k];
+  }
+}
+
+}  // namespace
+
 extern "C" {
 
 int32_t flk_default_params(flk_params *p) {
@@ -303,37 +344,7 @@ int32_t flk_linking_numbers(flk_ctx *ctx, const int32_t *pairs, int64_t npairs,
     } else {
       throw Error(FLK_ERR_ARG, "invalid method");
     }
-    // Fallback and judge: FP64 direct sum for every pair that lands farther
-    // than fallback_tol from an integer (or is not finite).
-    if (prm.fallback_tol > 0 && !(prm.method == FLK_DIRECTSUM && prm.fp64)) {
-      std::vector<int32_t> redo;
-      std::vector<int64_t> idx;
-      for (int64_t k = 0; k < npairs; ++k) {
-        const double d = std::fabs(v[k] - std::round(v[k]));
-        if (!(d <= prm.fallback_tol)) {
-          redo.push_back(pairs[2 * k]);
-          redo.push_back(pairs[2 * k + 1]);
-          idx.push_back(k);
-        }
-      }
-      if (!idx.empty()) {
-        std::vector<double> w;
-        float ms = 0.f;
-        flk::ds_run(ctx, redo.data(), (int64_t)idx.size(), true, w, &ms, prm.graph != 0);
-        ctx->stats.ms_fallback = ms;
-        ctx->stats.fallbacks = (int64_t)idx.size();
-        for (size_t t = 0; t < idx.size(); ++t) {
-          v[idx[t]] = w[t];
-          f[idx[t]] |= FLK_PAIR_FALLBACK;
-        }
-      }
-    }
-    for (int64_t k = 0; k < npairs; ++k) {
-      if (std::isnan(v[k])) f[k] |= FLK_PAIR_NAN;
-      lk[k] = v[k];
-      if (err_est) err_est[k] = e[k];
-      if (flags) flags[k] = f[k];
-    }
+    judge_and_finish(ctx, pairs, npairs, prm, v, e, f, lk, err_est, flags);
     ctx->stats.ms_total =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return FLK_OK;
@@ -458,6 +469,8 @@ int32_t flk_set_tune(flk_ctx *ctx, const flk_tune *tune) {
       throw Error(FLK_ERR_ARG, "bh_items_per_thread must be 2, 4 or 8");
     if (tune->ingest_threads < 0 || tune->ingest_threads > 64)
       throw Error(FLK_ERR_ARG, "ingest_threads must be in [0, 64]");
+    if (tune->bh_level_capacity != 0 && tune->bh_level_capacity < 64)
+      throw Error(FLK_ERR_ARG, "bh_level_capacity must be 0 (automatic) or at least 64");
     ctx->tune = *tune;
   })
   return FLK_OK;

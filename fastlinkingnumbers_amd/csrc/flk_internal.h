@@ -143,6 +143,8 @@ struct BhScratch {
   DevBuf roots_a, roots_b, beta2, cnt, leaves, pred, sum, err;
   std::vector<std::unique_ptr<DevBuf>> levels;
   unsigned long long *host_cnt = nullptr;  // pinned, 2 words per level of a traversal round trip
+  int64_t dev_cap = 0;                     // items per level buffer of the device-scheduled pass
+  std::vector<unsigned long long> dev_counts;  // its per-level counts, read back once per pass
   template <typename Fn> void each(Fn fn) {
     DevBuf *all[] = {&geo, &mom, &order, &m64, &range, &child, &flag, &scan, &keys0, &keys1, &ids0,
                      &tmp, &seg, &vtx, &box, &sp, &roots_a, &roots_b, &beta2, &cnt, &leaves, &pred,
@@ -229,7 +231,7 @@ struct flk_ctx {
   flk::HostStage stage;
   // default shapes: measured best on the 1,600-loop chain mail (R = 2 with
   // every block resident: 880 ms against 931 ms for R = 4, 1,068 ms for R = 8)
-  flk_tune tune{2, 0, 8, 4, 0, {0, 0, 0}};
+  flk_tune tune{2, 0, 8, 4, 0, 0, {0, 0}};
   flk::DsGraph ds_graph[2];      // FP32, FP64 batches
   flk::DiscResult disc;          // last flk_discretize result
   flk::BccLoad bcc;              // last flk_load_bcc result
@@ -309,8 +311,10 @@ void mlp_forward_device(flk_ctx *ctx, const float *dx, int64_t n, float *dy);
 struct BhItem;
 struct BhGeo;
 // n items, or with n_dev the count *n_dev (read on the device) bounded by n
+// (+inf where no network is needed); *evals counts the network evaluations.
 void mlp_predict_items(flk_ctx *ctx, const BhItem *items, int64_t n, const unsigned long long *n_dev,
-                       const BhGeo *geo, float eps, float *pred, hipEvent_t start, hipEvent_t stop);
+                       const BhGeo *geo, float eps, float *pred, unsigned long long *evals, hipEvent_t start,
+                       hipEvent_t stop);
 float mlp_gradient(flk_ctx *ctx, const float *x, const float *y, int64_t n, float *grad);
 float mlp_train(flk_ctx *ctx, const float *x, const float *y, int64_t n, int epochs, int batch,
                 float lr, uint64_t seed);

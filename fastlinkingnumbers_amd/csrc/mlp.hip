@@ -160,14 +160,29 @@ __device__ void stage_x(const BlockLds &L, WaveLds &W, const float *__restrict__
 // The A fragments of both layers live in VGPRs for the whole kernel, the
 // biases, output weights and input normalisation in LDS; activations never
 // leave registers.  One tile = 16 samples = 12 MFMAs per wave.
+//
+// Base-2 form: with z' = z log2 e, SiLU(z) = ln 2 * z' / (1 + 2^-z'), so the
+// layers run on W1' = W1 log2 e, b1' = b1 log2 e (giving z1'), h' = z' /
+// (1 + 2^-z') (= h / ln 2), then W2 h' + b2 log2 e = z2' and y = (w3 ln 2) h2'
+// + b3: an activation is one hardware 2^x, an add, a reciprocal and a
+// multiply, and the biases enter as the MFMA accumulator's initial value.
+constexpr float kLog2e = 1.44269504088896340736f;
+constexpr float kLn2 = 0.69314718055994530942f;
 struct InferRegs {
   bf16x8 a1[4], a2[4][2];
 };
 struct InferLds {
-  float b1[H], b2[H], w3[H];
-  float mean[16], istd[16];  // features 12..15 pad: mean 0, istd 0
+  float b1[H], b2[H], w3[H];  // b1 log2 e, b2 log2 e, w3 ln 2
+  float mean[16], istd[16];    // features 12..15 pad: mean 0, istd 0
+  float mean2[16], istd2[16];  // the same for log2 features
   float b3;
 };
+
+// h' = z' / (1 + 2^-z') = SiLU(z' ln 2) / ln 2 (hardware 2^x and reciprocal:
+// the activations are rounded to bf16 or summed into an FP32 output).
+__device__ __forceinline__ float silu2(float z) {
+  return z * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-z));
+}
 
 __device__ void load_infer(const float *__restrict__ P, const float *__restrict__ norm, InferRegs &R,
                            InferLds &L) {
@@ -177,7 +192,7 @@ __device__ void load_infer(const float *__restrict__ P, const float *__restrict_
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const int f = 8 * g + j;
-      R.a1[nb][j] = (__bf16)(f < FLK_MLP_NFEAT ? P[OFF_W1 + (nb * 16 + c) * FLK_MLP_NFEAT + f] : 0.0f);
+      R.a1[nb][j] = (__bf16)(f < FLK_MLP_NFEAT ? kLog2e * P[OFF_W1 + (nb * 16 + c) * FLK_MLP_NFEAT + f] : 0.0f);
     }
 #pragma unroll
     for (int st = 0; st < 2; ++st)
@@ -188,13 +203,16 @@ __device__ void load_infer(const float *__restrict__ P, const float *__restrict_
       }
   }
   for (int t = threadIdx.x; t < H; t += blockDim.x) {
-    L.b1[t] = P[OFF_B1 + t];
-    L.b2[t] = P[OFF_B2 + t];
-    L.w3[t] = P[OFF_W3 + t];
+    L.b1[t] = kLog2e * P[OFF_B1 + t];
+    L.b2[t] = kLog2e * P[OFF_B2 + t];
+    L.w3[t] = kLn2 * P[OFF_W3 + t];
   }
   for (int t = threadIdx.x; t < 16; t += blockDim.x) {
     L.mean[t] = t < FLK_MLP_NFEAT ? norm[t] : 0.f;
     L.istd[t] = t < FLK_MLP_NFEAT ? 1.0f / norm[FLK_MLP_NFEAT + t] : 0.f;
+    // features given as log2 values (bh_mlp_predict): (log2 x - mean / ln 2) * (istd ln 2)
+    L.mean2[t] = t < FLK_MLP_NFEAT ? norm[t] * kLog2e : 0.f;
+    L.istd2[t] = t < FLK_MLP_NFEAT ? kLn2 / norm[FLK_MLP_NFEAT + t] : 0.f;
   }
   if (threadIdx.x == 0) L.b3 = P[OFF_B3];
   __syncthreads();
@@ -208,28 +226,21 @@ __device__ __forceinline__ float infer_tile(const InferRegs &R, const InferLds &
   bf16x8 bh[2];
 #pragma unroll
   for (int nb = 0; nb < 4; ++nb) {
-    const f32x4 z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.a1[nb], bx, f32x4{0, 0, 0, 0}, 0, 0, 0);
     const float4 bb = *reinterpret_cast<const float4 *>(&L.b1[nb * 16 + 4 * g]);
-    const float bv[4] = {bb.x, bb.y, bb.z, bb.w};
+    const f32x4 z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.a1[nb], bx, f32x4{bb.x, bb.y, bb.z, bb.w}, 0, 0, 0);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const float u = z[i] + bv[i];
-      bh[nb >> 1][(nb & 1) * 4 + i] = (__bf16)(u * sigm(u));
-    }
+    for (int i = 0; i < 4; ++i) bh[nb >> 1][(nb & 1) * 4 + i] = (__bf16)silu2(z[i]);
   }
   float part = 0.f;
 #pragma unroll
   for (int nb = 0; nb < 4; ++nb) {
-    f32x4 z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.a2[nb][0], bh[0], f32x4{0, 0, 0, 0}, 0, 0, 0);
-    z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.a2[nb][1], bh[1], z, 0, 0, 0);
     const float4 bb = *reinterpret_cast<const float4 *>(&L.b2[nb * 16 + 4 * g]);
+    f32x4 z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.a2[nb][0], bh[0], f32x4{bb.x, bb.y, bb.z, bb.w}, 0, 0, 0);
+    z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.a2[nb][1], bh[1], z, 0, 0, 0);
     const float4 ww = *reinterpret_cast<const float4 *>(&L.w3[nb * 16 + 4 * g]);
-    const float bv[4] = {bb.x, bb.y, bb.z, bb.w}, wv[4] = {ww.x, ww.y, ww.z, ww.w};
+    const float wv[4] = {ww.x, ww.y, ww.z, ww.w};
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const float u = z[i] + bv[i];
-      part = fmaf(wv[i], u * sigm(u), part);
-    }
+    for (int i = 0; i < 4; ++i) part = fmaf(wv[i], silu2(z[i]), part);
   }
   part += __shfl_xor(part, 16, 64);
   part += __shfl_xor(part, 32, 64);
@@ -273,13 +284,18 @@ __global__ __launch_bounds__(256) void bh_mlp_predict(const BhItem *__restrict__
                                                       const BhGeo *__restrict__ geo, float eps,
                                                       const float *__restrict__ P,
                                                       const float *__restrict__ norm,
-                                                      float *__restrict__ pred) {
+                                                      float *__restrict__ pred,
+                                                      unsigned long long *__restrict__ evals) {
   __shared__ InferLds L;
   __shared__ __attribute__((aligned(16))) __bf16 xs[4][64][16];
   __shared__ int32_t src[4][64];
+  __shared__ unsigned int block_evals;  // network evaluations of the block
+  if (n_dev) n = level_count(n_dev, n);     // level enqueued ahead: n is the bound
+  if (n <= 0) return;                      // the whole grid, before any barrier
+  if (threadIdx.x == 0) block_evals = 0;
   InferRegs R;
-  load_infer(P, norm, R, L);
-  if (n_dev) n = min(n, (int64_t)*n_dev);  // level enqueued ahead: n is the bound
+  load_infer(P, norm, R, L);  // ends with a block barrier
+  unsigned int wave_evals = 0;
   const int l = threadIdx.x & 63, w = threadIdx.x >> 6, g = l >> 4, c = l & 15;
   const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + w;
   const int64_t stride = (int64_t)gridDim.x * (blockDim.x >> 6) * 64;
@@ -296,15 +312,15 @@ __global__ __launch_bounds__(256) void bh_mlp_predict(const BhItem *__restrict__
       need = !(A.c0 < 0 && B.c0 < 0) && dsq > rs * rs + eps;
       if (need) {
         float f[FLK_MLP_NFEAT];
-        mlp_features(A, B, dsq, f);
+        mlp_features_t<true>(A, B, dsq, f);  // log2 features, normalised with mean2 / istd2
         const unsigned long long m = __ballot(true);  // lanes with need == true
         const int pos = __popcll(m & ((1ull << l) - 1ull));
         bf16x8 v0, v1;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) v0[j] = (__bf16)((f[j] - L.mean[j]) * L.istd[j]);
+        for (int j = 0; j < 8; ++j) v0[j] = (__bf16)((f[j] - L.mean2[j]) * L.istd2[j]);
 #pragma unroll
         for (int j = 0; j < 8; ++j)
-          v1[j] = (__bf16)(j < FLK_MLP_NFEAT - 8 ? (f[8 + j] - L.mean[8 + j]) * L.istd[8 + j] : 0.f);
+          v1[j] = (__bf16)(j < FLK_MLP_NFEAT - 8 ? (f[8 + j] - L.mean2[8 + j]) * L.istd2[8 + j] : 0.f);
         *reinterpret_cast<bf16x8 *>(&xs[w][pos][0]) = v0;
         *reinterpret_cast<bf16x8 *>(&xs[w][pos][8]) = v1;
         src[w][pos] = l;
@@ -313,6 +329,7 @@ __global__ __launch_bounds__(256) void bh_mlp_predict(const BhItem *__restrict__
       }
     }
     const int cnt = __popcll(__ballot(need));
+    wave_evals += (unsigned int)cnt;
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     // next iteration's records, in flight while the tiles run
@@ -335,6 +352,9 @@ __global__ __launch_bounds__(256) void bh_mlp_predict(const BhItem *__restrict__
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   }
+  if (l == 0 && wave_evals) atomicAdd(&block_evals, wave_evals);
+  __syncthreads();
+  if (threadIdx.x == 0 && block_evals) atomicAdd(evals, (unsigned long long)block_evals);
 }
 
 // Forward + backward of the mean squared error over rows idx[0..n) (or 0..n):
@@ -543,11 +563,12 @@ void mlp_forward_device(flk_ctx *ctx, const float *dx, int64_t n, float *dy) {
 }
 
 void mlp_predict_items(flk_ctx *ctx, const BhItem *items, int64_t n, const unsigned long long *n_dev,
-                       const BhGeo *geo, float eps, float *pred, hipEvent_t start, hipEvent_t stop) {
+                       const BhGeo *geo, float eps, float *pred, unsigned long long *evals, hipEvent_t start,
+                       hipEvent_t stop) {
   if (n <= 0) return;
   const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8 * std::max(1, ctx->num_cus)));
   launch_timed(start, stop, bh_mlp_predict, dim3(blocks), dim3(256), ctx->stream, items, n, n_dev, geo, eps,
-               ctx->mlp.d_params.as<float>(), ctx->mlp.d_norm.as<float>(), pred);
+               ctx->mlp.d_params.as<float>(), ctx->mlp.d_norm.as<float>(), pred, evals);
   FLK_HIP(hipGetLastError());
 }
 

@@ -87,7 +87,11 @@ typedef struct flk_stats {
   int32_t ds_waves;         /* waves per direct-sum launch */
   double ms_bh_level;       /* device time of the traversal level kernels (params.timing) */
   double ms_mlp;            /* device time of the learned-criterion predictions (params.timing) */
-  int64_t reserved[6];
+  int64_t mlp_evals;        /* learned-criterion network evaluations */
+  int64_t bh_host_passes;   /* Barnes–Hut passes that ran level by level from the host
+                               (per-launch timing, or the device-scheduled pass's
+                               buffers overflowed) instead of all levels enqueued at once */
+  int64_t reserved[4];
 } flk_stats;
 
 /* Fills *p with the reference defaults (method Barnes–Hut, beta 2/10, ...). */
@@ -251,7 +255,11 @@ typedef struct flk_tune {
   int32_t bh_items_per_thread; /* Barnes–Hut level and leaf kernels: 2, 4 or 8 */
   int32_t ingest_threads;      /* host threads filling the upload's pinned
                                   staging buffers (0: half the cores, at most 8) */
-  int32_t reserved[3];
+  int32_t bh_level_capacity;   /* Barnes–Hut items per level buffer when all levels
+                                  of a pass are enqueued at once (0: sized from free
+                                  memory, at most 2^26); a pass whose level or leaf
+                                  list outgrows it re-runs level by level from the host */
+  int32_t reserved[2];
 } flk_tune;
 
 int32_t flk_get_tune(const flk_ctx *ctx, flk_tune *out);

@@ -45,6 +45,29 @@ def forward(P, norm, X, emulate=False):
     return y, (xn, z1, h1, z2)
 
 
+LOG2E, LN2 = 1.4426950408889634, 0.6931471805599453
+
+
+def forward_infer(P, norm, X, rounding=True):
+    """The inference kernel's base-2 form with its bf16 operands: z' = z log2 e
+    from W1 log2 e and b1 log2 e, h' = z' / (1 + 2^-z') = SiLU(z) / ln 2,
+    z2' = W2 h' + b2 log2 e, y = (w3 ln 2) . h2' + b3 -- in exact arithmetic
+    (rounding=False: float64 throughout) the same function as forward()."""
+    W1, b1, W2, b2, w3, b3 = unpack(P)
+    if rounding:
+        r, f = bf16, np.float32
+        xn = (np.asarray(X, np.float32) - np.float32(norm[:NF])) * (np.float32(1) / np.float32(norm[NF:]))
+    else:
+        r, f = (lambda a: np.asarray(a, dtype=np.float64)), np.float64
+        xn = (np.asarray(X, np.float64) - norm[:NF]) / norm[NF:]
+    xn = r(xn)
+    silu2 = lambda z: z / (1 + np.exp2(-z))  # noqa: E731
+    z1 = xn @ r(f(LOG2E) * W1.astype(f)).T + f(LOG2E) * b1.astype(f)
+    h1 = r(silu2(z1))
+    z2 = h1 @ r(W2).T + f(LOG2E) * b2.astype(f)
+    return silu2(z2) @ (f(LN2) * w3.astype(f)) + b3
+
+
 def loss_grad(P, norm, X, T, emulate=False):
     W1, b1, W2, b2, w3, b3 = unpack(P)
     r = bf16 if emulate else (lambda a: np.asarray(a, dtype=np.float64))

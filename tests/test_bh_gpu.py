@@ -120,6 +120,34 @@ def test_bh_lattice_hybrid_batches(engine):
 
 
 @pytest.mark.parametrize("criterion", ["beta", "mlp"])
+def test_device_schedule_and_overflow_fallback(engine, criterion):
+    """A pass normally enqueues every level at once (counts read on the
+    device, one read-back per pass).  With level buffers too small for this
+    pair (forced from the host through bh_level_capacity) a level overflows:
+    the later levels skip their work, and the pass re-runs level by level
+    from the host, visiting exactly the same node pairs."""
+    from fastlinkingnumbers_amd import criterion as crit
+    engine.mlp_set_weights(*crit.load_weights())
+    curves = W.random_walk_pair(seed=1, n=20000, rg=30.0, workload=3)
+    engine.set_curves(curves)
+    kw = dict(method="barneshut", criterion=criterion, fallback_tol=0.0)
+    base = engine.linking_numbers([[0, 1]], **kw)
+    assert base.stats["bh_host_passes"] == 0
+    assert (base.stats["mlp_evals"] > 0) == (criterion == "mlp")
+    engine.set_tune(bh_level_capacity=4096)
+    small = engine.linking_numbers([[0, 1]], **kw)
+    print(criterion, "visits", base.stats["bh_visits"], "host passes", small.stats["bh_host_passes"],
+          "lk", base.lk[0], small.lk[0])
+    assert small.stats["bh_host_passes"] >= 1
+    for key in ("bh_visits", "bh_far", "bh_leaf_pairs", "retries", "mlp_evals"):
+        assert small.stats[key] == base.stats[key], key
+    np.testing.assert_allclose(small.lk, base.lk, rtol=0, atol=1e-9)
+    np.testing.assert_allclose(small.err, base.err, rtol=1e-9, atol=0)
+    with pytest.raises(flk.FlkError, match="bh_level_capacity"):
+        engine.set_tune(bh_level_capacity=8)
+
+
+@pytest.mark.parametrize("criterion", ["beta", "mlp"])
 def test_bh_kernel_timing_mode(engine, criterion):
     """params.timing = 1 times every level (and prediction) launch with HIP
     events without changing what is computed.  A timed traversal also reads

@@ -45,3 +45,44 @@ def test_bench_refuses_several_gpus():
     assert r.returncode == 2, r.stdout + r.stderr
     assert "one GPU" in r.stdout + r.stderr
     assert r.stdout.strip() == "" or not r.stdout.lstrip().startswith("{")
+
+
+def _bench_module():
+    import importlib.util
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(root, "bench.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def test_fan_pieces_sum_to_the_whole_curve():
+    """The FP64 reference of the 2 x 8M pair splits curve 0 into closed
+    sub-loops through vertex 0; their linking integrals with the other curve
+    must add up to the whole curve's (the chords cancel), checked with the
+    oracle's FP64 direct sum."""
+    from fastlinkingnumbers_amd import workloads as W
+    bench = _bench_module()
+    a, b = W.random_walk_pair(seed=3, n=700, rg=6.0, workload=5)
+    whole = O.ds(a, b)
+    for k in (2, 5, 16):
+        pieces = bench.fan_pieces(a, k)
+        assert len(pieces) == k and sum(len(p) for p in pieces) == len(a) + 2 * (k - 1)
+        assert abs(sum(O.ds(p, b) for p in pieces) - whole) < 1e-10
+
+
+def test_bench_cpu_worker_runs_the_oracle():
+    """The CPU baseline's child process: oracle only (no device), one JSON
+    line with the run times and the pair sums, equal to the oracle's."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    spec = {"kind": "pairs", "workload": 2, "scale": 0.01, "reps": 2, "pairs": [[0, 1], [0, 2]]}
+    import json
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--cpu-worker", json.dumps(spec)],
+                       capture_output=True, text=True, timeout=300, cwd=root)
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    assert len(out["times"]) == 2 and len(out["lk"]) == 2
+    from fastlinkingnumbers_amd import workloads as W
+    curves = W.workload(2, scale=0.01)
+    xyz, offs = flk.pack_curves(curves)
+    np.testing.assert_allclose(out["lk"], O.ds_pairs(xyz, offs, np.array([[0, 1], [0, 2]], np.int32)), atol=1e-12)

@@ -33,3 +33,19 @@ def test_weight_file_roundtrip(tmp_path):
     criterion.save_weights(p, P, nm)
     P2, n2 = criterion.load_weights(p)
     assert np.array_equal(P, P2) and np.array_equal(nm, n2)
+
+
+def test_base2_inference_form_is_the_same_network():
+    """The inference kernel's base-2 rewrite (log2 e folded into layer 1 and
+    the biases, ln 2 into the output weights, activations z / (1 + 2^-z)) is
+    the SiLU network itself in exact arithmetic."""
+    g = np.random.default_rng(2)
+    P = M.init_params(7).astype(np.float64)
+    P[768:832] = g.normal(scale=0.3, size=64)
+    P[4928:4992] = g.normal(scale=0.3, size=64)
+    P[5056] = -1.5
+    norm = np.concatenate([g.normal(size=12), g.uniform(0.5, 2, size=12)])
+    X = g.normal(size=(50, 12)) * norm[12:] + norm[:12]
+    y, _ = M.forward(P, norm, X)
+    y2 = M.forward_infer(P, norm, X, rounding=False)
+    np.testing.assert_allclose(y2, y, rtol=1e-12, atol=1e-12)

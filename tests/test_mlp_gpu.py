@@ -27,13 +27,20 @@ def _setup(engine, seed=3):
 
 
 def test_mlp_forward_matches_numpy(engine):
+    """The inference kernel runs the network in base 2 (log2 e folded into the
+    first layer and the biases, ln 2 into the output weights); the numpy
+    model of that form with the same bf16 operands must agree to 2e-3, and
+    the exact float64 network (a different rounding of the same function)
+    to the bf16 rounding level."""
     P, norm, g = _setup(engine)
     X = (g.normal(size=(203, 12)) * norm[12:] + norm[:12]).astype(np.float32)
     y = engine.mlp_forward(X)
-    ref, _ = M.forward(P, norm.astype(np.float64), X, emulate=True)
+    ref = M.forward_infer(P, norm.astype(np.float64), X)
+    exact, _ = M.forward(P, norm.astype(np.float64), X)
     err = np.abs(y - ref)
-    print("forward max err", err.max())
+    print("forward max err", err.max(), "vs exact float64", np.abs(y - exact).max())
     assert np.all(err <= 2e-3 * (1 + np.abs(ref)))
+    assert np.all(np.abs(y - exact) <= 3e-2 * (1 + np.abs(exact)))
 
 
 def test_mlp_gradient_matches_cpu(engine):
