@@ -163,6 +163,9 @@ FLK_EXPORTS = [
     ("flk_linking_numbers", C.c_int32,
      [C.c_void_p, _P(C.c_int32), C.c_int64, _P(FlkParams), _P(C.c_double), _P(C.c_double),
       _P(C.c_uint32)]),
+    ("flk_linking_numbers_noalloc", C.c_int32,
+     [C.c_void_p, _P(C.c_int32), C.c_int64, _P(FlkParams), _P(C.c_double), _P(C.c_double),
+      _P(C.c_uint32)]),
     ("flk_get_stats", C.c_int32, [C.c_void_p, _P(FlkStats)]),
     ("flk_tree_moments", C.c_int32,
      [C.c_void_p, C.c_int32, C.c_int32, _P(FlkTreeNode), C.c_int32, _P(C.c_int32), _P(C.c_int32)]),
@@ -370,6 +373,24 @@ class Engine:
                                           _ptr(lk, C.c_double), _ptr(err, C.c_double),
                                           _ptr(flags, C.c_uint32))
         self._check(rc, "flk_linking_numbers")
+        return LinkResult(lk, err, flags, self.stats())
+
+    def linking_numbers_noalloc(self, pairs: np.ndarray, params: FlkParams | None = None, **kw) -> LinkResult:
+        """Direct sum through flk_linking_numbers_noalloc: no device
+        allocation, the batch's arrays in the library's fixed per-device
+        array (at most 65,536 pairs and 1,048,576 tasks per call)."""
+        pairs = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1, 2))
+        m = len(pairs)
+        if params is None:
+            kw.setdefault("method", "directsum")
+            params = default_params(**kw)
+        lk = np.zeros(m, dtype=np.float64)
+        err = np.zeros(m, dtype=np.float64)
+        flags = np.zeros(m, dtype=np.uint32)
+        rc = self.lib.flk_linking_numbers_noalloc(self.h, _ptr(pairs, C.c_int32), m, C.byref(params),
+                                                  _ptr(lk, C.c_double), _ptr(err, C.c_double),
+                                                  _ptr(flags, C.c_uint32))
+        self._check(rc, "flk_linking_numbers_noalloc")
         return LinkResult(lk, err, flags, self.stats())
 
     def tree_moments(self, curve: int, leaf_size: int = 1) -> tuple[dict, np.ndarray]:

@@ -36,6 +36,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <mutex>
 
 #include "flk_internal.h"
 
@@ -702,6 +703,79 @@ void ds_run(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, bool fp64,
   }
   if (ms_kernel) *ms_kernel = ms;
   ctx->stats.ds_seg_pairs += work;
+}
+
+// ---------------------------------------------------------------- without allocation
+// One batch's device arrays (pair records, per-pair first task and sum, the
+// tasks, their cost prefix and partials) carved from a fixed array, for
+// callers that cannot allocate.  Every task writes its own partial and the
+// reduction writes every pair's sum, so nothing needs clearing between calls.
+namespace {
+constexpr size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
+constexpr size_t kNaPairs = 0;
+constexpr size_t kNaTask0 = align256(kNaPairs + sizeof(DsPair) * FLK_NOALLOC_MAX_PAIRS);
+constexpr size_t kNaOut = align256(kNaTask0 + sizeof(int64_t) * (FLK_NOALLOC_MAX_PAIRS + 1));
+constexpr size_t kNaTasks = align256(kNaOut + sizeof(double) * FLK_NOALLOC_MAX_PAIRS);
+constexpr size_t kNaCost = align256(kNaTasks + sizeof(DsTask) * FLK_NOALLOC_MAX_TASKS);
+constexpr size_t kNaTheta = align256(kNaCost + sizeof(int64_t) * (FLK_NOALLOC_MAX_TASKS + 1));
+constexpr size_t kNaCorr = align256(kNaTheta + sizeof(double) * FLK_NOALLOC_MAX_TASKS);
+constexpr size_t kNaBytes = align256(kNaCorr + sizeof(int32_t) * FLK_NOALLOC_MAX_TASKS);
+static_assert(kNaBytes <= (size_t)FLK_NOALLOC_SCRATCH_MB << 20, "scratch layout exceeds its stated size");
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// Each GPU holds its own instance of this array; host code reaches it only
+// through the per-device address from hipGetSymbolAddress (the array's name
+// is a host-side stand-in in host code).
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+void *noalloc_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) throw Error(FLK_ERR_DEVICE, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw Error(FLK_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = p;
+  }
+  return g_scratch_ptr[device];
+}
+
+void ds_run_noalloc(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, bool fp64, std::vector<double> &lk_out,
+                    float *ms_kernel) {
+  lk_out.assign(npairs, 0.0);
+  if (ms_kernel) *ms_kernel = 0.f;
+  if (npairs == 0) return;
+  if (npairs > FLK_NOALLOC_MAX_PAIRS)
+    throw Error(FLK_ERR_ARG, "a batch without allocation holds at most " + std::to_string(FLK_NOALLOC_MAX_PAIRS) +
+                                 " pairs (got " + std::to_string(npairs) + ")");
+  unsigned char *base = static_cast<unsigned char *>(ctx->noalloc_scratch);
+  if (!base) throw Error(FLK_ERR_STATE, "no scratch array on this device");
+  DsPlan P;
+  ds_plan(ctx, pairs, npairs, fp64, ctx->tune, 1, P);
+  const int64_t nt = (int64_t)P.tasks.size();
+  if (nt > FLK_NOALLOC_MAX_TASKS)
+    throw Error(FLK_ERR_ARG, "a batch without allocation holds at most " + std::to_string(FLK_NOALLOC_MAX_TASKS) +
+                                 " tasks (this one needs " + std::to_string(nt) + ")");
+  ctx->stats.ds_seg_pairs += P.work;
+  ctx->stats.ds_launches += 1;
+  ctx->stats.ds_waves = P.blocks * 4;
+  if (nt == 0) return;
+  const DsBufs d{reinterpret_cast<DsPair *>(base + kNaPairs), reinterpret_cast<DsTask *>(base + kNaTasks),
+                 reinterpret_cast<int64_t *>(base + kNaCost),  reinterpret_cast<int64_t *>(base + kNaTask0),
+                 reinterpret_cast<double *>(base + kNaTheta),  reinterpret_cast<double *>(base + kNaOut),
+                 reinterpret_cast<int32_t *>(base + kNaCorr)};
+  hipStream_t s = ctx->stream;
+  ds_enqueue(ctx, P, P.dp.data(), P.tasks.data(), P.prefix.data(), P.task0.data(), npairs, nt, d, s, ctx->ev0,
+             ctx->ev1, lk_out.data());
+  FLK_HIP(hipStreamSynchronize(s));
+  float ms = 0.f;
+  FLK_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  if (ms_kernel) *ms_kernel = ms;
 }
 
 float ds_time_sample(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, const flk_tune &t, int64_t stride,

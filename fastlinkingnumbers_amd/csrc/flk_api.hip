@@ -133,10 +133,11 @@ namespace {
 
 // Fallback and judge: the FP64 direct sum replaces every pair that lands
 // farther than fallback_tol from an integer (or is not finite); then the NaN
-// flags and the caller's outputs.
+// flags and the caller's outputs.  `noalloc` keeps the recomputation in the
+// fixed direct-sum array.
 void judge_and_finish(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, const flk_params &prm,
                       std::vector<double> &v, const std::vector<double> &e, std::vector<uint32_t> &f, double *lk,
-                      double *err_est, uint32_t *flags) {
+                      double *err_est, uint32_t *flags, bool noalloc) {
   if (prm.fallback_tol > 0 && !(prm.method == FLK_DIRECTSUM && prm.fp64)) {
     std::vector<int32_t> redo;
     std::vector<int64_t> idx;
@@ -151,7 +152,8 @@ void judge_and_finish(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, const 
     if (!idx.empty()) {
       std::vector<double> w;
       float ms = 0.f;
-      flk::ds_run(ctx, redo.data(), (int64_t)idx.size(), true, w, &ms, prm.graph != 0);
+      if (noalloc) flk::ds_run_noalloc(ctx, redo.data(), (int64_t)idx.size(), true, w, &ms);
+      else flk::ds_run(ctx, redo.data(), (int64_t)idx.size(), true, w, &ms, prm.graph != 0);
       ctx->stats.ms_fallback = ms;
       ctx->stats.fallbacks = (int64_t)idx.size();
       for (size_t t = 0; t < idx.size(); ++t) {
@@ -213,6 +215,7 @@ int32_t flk_create(int32_t device, flk_ctx **out) {
     FLK_HIP(hipEventCreate(&ctx->ev1));
     FLK_HIP(hipEventCreate(&ctx->ev2));
     FLK_HIP(hipEventCreate(&ctx->ev3));
+    ctx->noalloc_scratch = flk::noalloc_scratch_init(device);  // once per device, this device current
   } catch (const flk::Error &e) {
     g_create_err = e.msg;
     delete ctx;
@@ -344,7 +347,38 @@ int32_t flk_linking_numbers(flk_ctx *ctx, const int32_t *pairs, int64_t npairs,
     } else {
       throw Error(FLK_ERR_ARG, "invalid method");
     }
-    judge_and_finish(ctx, pairs, npairs, prm, v, e, f, lk, err_est, flags);
+    judge_and_finish(ctx, pairs, npairs, prm, v, e, f, lk, err_est, flags, false);
+    ctx->stats.ms_total =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return FLK_OK;
+  })
+}
+
+int32_t flk_linking_numbers_noalloc(flk_ctx *ctx, const int32_t *pairs, int64_t npairs,
+                                    const flk_params *params, double *lk, double *err_est,
+                                    uint32_t *flags) {
+  if (!ctx || npairs < 0 || (npairs > 0 && (!pairs || !lk))) return FLK_ERR_ARG;
+  FLK_GUARD(ctx, {
+    FLK_HIP(hipSetDevice(ctx->device));
+    flk_params prm;
+    if (params) prm = *params;
+    else flk_default_params(&prm);
+    if (prm.method != FLK_DIRECTSUM)
+      throw Error(FLK_ERR_ARG, "flk_linking_numbers_noalloc runs the direct sum only (method FLK_DIRECTSUM)");
+    const auto t0 = std::chrono::steady_clock::now();
+    std::memset(&ctx->stats, 0, sizeof ctx->stats);
+    const flk::Curves &cv = ctx->curves;
+    for (int64_t k = 0; k < 2 * npairs; ++k)
+      if (pairs[k] < 0 || pairs[k] >= cv.ncurves) throw Error(FLK_ERR_ARG, "pair index out of range");
+    for (int64_t k = 0; k < npairs; ++k)
+      ctx->stats.seg_pairs += (int64_t)cv.n[pairs[2 * k]] * cv.n[pairs[2 * k + 1]];
+    std::vector<double> v, e(npairs, 0.0);
+    std::vector<uint32_t> f(npairs, 0u);
+    float ms = 0.f;
+    flk::ds_run_noalloc(ctx, pairs, npairs, prm.fp64 != 0, v, &ms);
+    ctx->stats.ms_kernel = ms;
+    ctx->stats.ms_ds_kernel = ms;
+    judge_and_finish(ctx, pairs, npairs, prm, v, e, f, lk, err_est, flags, true);
     ctx->stats.ms_total =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return FLK_OK;

@@ -237,6 +237,7 @@ struct flk_ctx {
   flk::BccLoad bcc;              // last flk_load_bcc result
   flk::DevBuf bcc_land;          // FP32 landing area of flk_load_bcc
   uint64_t curves_version = 0;   // bumped by every flk_set_curves
+  void *noalloc_scratch = nullptr;  // this device's fixed direct-sum array (ds.hip)
   // scratch
   flk::DevBuf d_pairs, d_tasks, d_cost, d_task_theta, d_task_corr, d_pair_task0,
       d_pair_out, d_pair_corr, d_boxes, d_pls, d_count, d_misc;
@@ -256,6 +257,13 @@ float ds_time_sample(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, const f
                      int64_t *work);
 // Segment pairs of the pairs' direct-sum work.
 int64_t ds_work(flk_ctx *ctx, const int32_t *pairs, int64_t npairs);
+// Device address of this GPU's fixed direct-sum array (looked up once per
+// device; call with the device current).
+void *noalloc_scratch_init(int device);
+// ds_run launched eagerly with every device array carved from the fixed
+// array: no device allocation (limits FLK_NOALLOC_MAX_PAIRS / _TASKS).
+void ds_run_noalloc(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, bool fp64, std::vector<double> &lk_out,
+                    float *ms_kernel);
 
 // Spline-to-polyline discretization of the given curves against the given
 // potentially linked pairs (see disc.hip); the polylines become the context's

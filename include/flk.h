@@ -146,6 +146,27 @@ int32_t flk_linking_numbers(flk_ctx *ctx, const int32_t *pairs, int64_t npairs,
                             const flk_params *params, double *lk, double *err_est,
                             uint32_t *flags);
 
+/* No counterpart in the reference: flk_linking_numbers for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Direct summation only (params->method
+ * must be FLK_DIRECTSUM; FP32, or FP64 with params->fp64), with the same
+ * kernels, results and FP64 judge as flk_linking_numbers launched eagerly
+ * (params->graph is ignored).  The batch's plan, per-task partials and
+ * per-pair sums live in one fixed device array per GPU of
+ * FLK_NOALLOC_SCRATCH_MB MiB, held from the first flk_create on that GPU until
+ * the process ends; a batch may hold at most FLK_NOALLOC_MAX_PAIRS pairs and
+ * FLK_NOALLOC_MAX_TASKS tasks (the plan has one task per 64R rows and column
+ * chunk; FLK_ERR_ARG otherwise, naming the count).  The curve set must have
+ * been uploaded before (flk_set_curves*, which allocates).  Every call on a
+ * GPU shares that GPU's array: calls that overlap on one GPU (two contexts
+ * driven from two host threads) race and must be serialised by the caller. */
+#define FLK_NOALLOC_MAX_PAIRS 65536
+#define FLK_NOALLOC_MAX_TASKS 1048576
+#define FLK_NOALLOC_SCRATCH_MB 40
+int32_t flk_linking_numbers_noalloc(flk_ctx *ctx, const int32_t *pairs, int64_t npairs,
+                                    const flk_params *params, double *lk, double *err_est,
+                                    uint32_t *flags);
+
 /* Statistics of the last flk_linking_numbers call. */
 int32_t flk_get_stats(const flk_ctx *ctx, flk_stats *out);
 
