@@ -108,11 +108,27 @@ void curves_finish(flk_ctx *ctx) {
     const double grid = std::ldexp(1.0, (int)std::floor(std::log2(extent)));
     for (int d = 0; d < 3; ++d) cv.origin[d] = grid * std::round(0.5 * (lo[d] + hi[d]) / grid);
   }
-  double span = 0.0;
+  // The scale is set by the largest curve, not by the model: two curves whose
+  // boxes overlap lie within twice the largest box diagonal of each other,
+  // so every relative vector of a candidate pair stays within length 1 (the
+  // FP32 direct sum's products of six of them, and of twelve in its paired
+  // short path, stay far from overflow), while segments keep their size
+  // relative to that of a curve.  Scaling by the model's span instead shrank a
+  // 160-unit-tall lattice of radius-1 rings by 256, and the sixth powers of
+  // its 1e-7 relative vectors underflowed (a pair's FP32 sum landed 0.33 off
+  // its integer).
+  double dmax = 0.0, span = 0.0;
+  for (int32_t c = 0; c < ncurves; ++c)
+    if (cv.box[6 * c] <= cv.box[6 * c + 3]) {
+      double dsq = 0.0;
+      for (int d = 0; d < 3; ++d) dsq += (cv.box[6 * c + 3 + d] - cv.box[6 * c + d]) * (cv.box[6 * c + 3 + d] - cv.box[6 * c + d]);
+      dmax = std::max(dmax, std::sqrt(dsq));
+    }
   for (int d = 0; d < 3; ++d)
     if (lo[d] <= hi[d])
       span = std::max(span, std::max(std::fabs(hi[d] - cv.origin[d]), std::fabs(lo[d] - cv.origin[d])));
-  cv.scale = span > 0 ? std::ldexp(1.0, -(int)std::ceil(std::log2(span))) : 1.0;
+  const double ref = dmax > 0 ? 2.0 * dmax : span;
+  cv.scale = ref > 0 ? std::ldexp(1.0, -(int)std::ceil(std::log2(ref))) : 1.0;
   if (!std::isfinite(cv.scale) || cv.scale == 0) cv.scale = 1.0;
   if (cv.nvert_dev > 0) {
     stage_vertices_launch(ctx);

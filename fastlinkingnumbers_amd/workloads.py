@@ -125,25 +125,31 @@ def random_walk_pair(seed: int = 0, n: int = 2_000_000, rg: float = 100.0, workl
 
 # ---------------------------------------------------------------- workload 4
 def helical_lattice(seed: int = 0, nloops: int = 10000, per_turn: int = 101, nmin: int = 64,
-                    nmax: int = 262144, pitch: float = 1.6, amp: float = 0.1) -> list[np.ndarray]:
+                    nmax: int = 262144, pitch: float = 1.6, amp: float = 0.1,
+                    only=None) -> list[np.ndarray]:
     """The chain-mail sheet wrapped into a tube: ring k sits at angle
     2 pi k / per_turn on a cylinder whose circumference is per_turn * pitch,
     rising pitch per turn, so ring k neighbours k +- 1 along the helix and
     k +- per_turn across turns (per_turn odd keeps the tilt alternating both
     ways).  Each ring carries 3 random Fourier modes (total amplitude <= amp).
     At pitch 1.6 a ring links 0-4 of its neighbours (mostly 2-3, LK +-1): the
-    Fourier modes unlink some neighbour pairs that pitch 1.5 keeps linked."""
+    Fourier modes unlink some neighbour pairs that pitch 1.5 keeps linked.
+    only: ring indices to build (in that order), the others' random draws
+    still consumed, so each ring is the one of the whole lattice."""
     g = _rng(4, seed)
     ns = _loglog_counts(g, nloops, nmin, nmax)
     Rt = per_turn * pitch / (2 * np.pi)
     Rp, Rm = _rot((1, 1, 0), np.pi / 4), _rot((1, 1, 0), -np.pi / 4)
-    out = []
+    want = None if only is None else set(int(k) for k in only)
+    out, built = [], {}
     for k in range(nloops):
         n = ns[k]
-        t = 2 * np.pi * np.arange(n) / n
         freq = g.integers(2, 7, size=(3, 2))
         ph = g.uniform(0, 2 * np.pi, size=(3, 2))
         am = g.uniform(0, amp / 3, size=(3, 2))
+        if want is not None and k not in want:
+            continue
+        t = 2 * np.pi * np.arange(n) / n
         r = 1 + sum(am[m, 0] * np.cos(freq[m, 0] * t + ph[m, 0]) for m in range(3))
         w = sum(am[m, 1] * np.sin(freq[m, 1] * t + ph[m, 1]) for m in range(3))
         p = np.stack([r * np.cos(t), r * np.sin(t), w], 1)
@@ -154,8 +160,12 @@ def helical_lattice(seed: int = 0, nloops: int = 10000, per_turn: int = 101, nmi
         e_z = np.array([0.0, 0.0, 1.0])
         e_r = np.array([np.cos(th), np.sin(th), 0.0])
         ctr = Rt * e_r + z * e_z
-        out.append(_f32(ctr + np.outer(p[:, 0], e_t) + np.outer(p[:, 1], e_z) + np.outer(p[:, 2], e_r)))
-    return out
+        ring = _f32(ctr + np.outer(p[:, 0], e_t) + np.outer(p[:, 1], e_z) + np.outer(p[:, 2], e_r))
+        if want is None:
+            out.append(ring)
+        else:
+            built[k] = ring
+    return out if want is None else [built[int(k)] for k in only]
 
 
 def workload(num: int, scale: float = 1.0, seed: int = 0) -> list[np.ndarray]:

@@ -74,3 +74,24 @@ def test_learned_workload_pair_matches_oracle(engine, criterion):
           f"visits {s['bh_visits']} retries {s['retries']}")
     assert round(res.lk[0]) == round(ref)
     assert abs(res.lk[0] - ref) < 0.05
+
+
+def test_near_approaches_in_a_tall_model(engine):
+    """The five pairs of the full-size helical lattice that pass closest (their
+    rings come within 3e-5 to 1e-4 of each other, one to three segment
+    lengths, with up to 246,473 vertices) inside a model 160 units tall: the
+    FP32 direct sum equals the FP64 direct sum.  With coordinates scaled by
+    the model's height the FP32 sixth powers of those relative vectors
+    underflowed and the sums landed up to 0.33 off their integers."""
+    idx = [814, 915, 2155, 2156, 1691, 1692, 6385, 6486, 3647, 3648]
+    rings = W.helical_lattice(only=idx)
+    t = np.linspace(0, 2 * np.pi, 32, endpoint=False)
+    cap = np.stack([np.cos(t), np.sin(t), 0 * t + 158.4], 1)  # the lattice's top, for its full height
+    engine.set_curves(rings + [cap])
+    pairs = np.arange(10, dtype=np.int32).reshape(5, 2)
+    r32 = engine.linking_numbers(pairs, method="directsum", fallback_tol=0.0)
+    r64 = engine.linking_numbers(pairs, method="directsum", fp64=1)
+    print("fp32", r32.lk, "fp64", r64.lk)
+    assert np.array_equal(np.round(r64.lk), [1, -1, -1, 0, -1])
+    assert np.max(np.abs(r64.lk - np.round(r64.lk))) < 1e-9
+    np.testing.assert_allclose(r32.lk, r64.lk, rtol=0, atol=1e-5)

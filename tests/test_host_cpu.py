@@ -86,3 +86,12 @@ def test_bench_cpu_worker_runs_the_oracle():
     curves = W.workload(2, scale=0.01)
     xyz, offs = flk.pack_curves(curves)
     np.testing.assert_allclose(out["lk"], O.ds_pairs(xyz, offs, np.array([[0, 1], [0, 2]], np.int32)), atol=1e-12)
+
+
+def test_lattice_subset_equals_the_whole():
+    """helical_lattice(only=...) builds the same rings as the whole lattice."""
+    from fastlinkingnumbers_amd import workloads as W
+    whole = W.helical_lattice(nloops=240, nmin=16, nmax=300)
+    part = W.helical_lattice(nloops=240, nmin=16, nmax=300, only=[200, 3, 101])
+    for k, ring in zip([200, 3, 101], part):
+        assert np.array_equal(ring, whole[k])
