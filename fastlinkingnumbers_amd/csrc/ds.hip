@@ -212,7 +212,7 @@ __global__ __launch_bounds__(256) void ds_tile_f32(
       // det of the couple (2c, 2c+1) from one cross product at grid point
       // 2c+1 (computed for all couples first: interleaving them with the
       // denominators measured 2% slower)
-      float xp[R], yp[R], s12[R], dt[R];
+      float xp[R], yp[R], s12[R], dt[R], d1[R], d2[R];
 #pragma unroll
       for (int c = 0; c < R / 2; ++c) {
         const int k1 = 2 * c + 1;
@@ -229,6 +229,8 @@ __global__ __launch_bounds__(256) void ds_tile_f32(
         const float s = fmaf(la, lc, diag[k]);
         const float div1 = fmaf(lb, s, fmaf(gcol[k], lc, row[k + 1] * la));
         const float div2 = fmaf(ld, s, fmaf(row[k], lc, ncol[k] * la));
+        d1[k] = div1;
+        d2[k] = div2;
         xp[k] = fmaf(div1, div2, -dt[k] * dt[k]);
         s12[k] = div1 + div2;
         yp[k] = dt[k] * s12[k];
@@ -269,7 +271,18 @@ __global__ __launch_bounds__(256) void ds_tile_f32(
         // path merges without register copies
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-          sacc += atan2_fast(yp[k], xp[k]);
+          float x = xp[k], y = yp[k];
+          // x' and y' are products of two factors of |G|^3 (denominators, and
+          // the triple product times their sum): for relative vectors below
+          // ~1e-6 of the unit they underflow FP32 although the factors do not;
+          // such a pair is redone from its factors scaled by 2^40 (the angle
+          // depends on the ratio only).  Only this rare path pays for it.
+          if (fabsf(x) < 0x1p-100f && fabsf(y) < 0x1p-100f) {
+            const float a = d1[k] * 0x1p40f, b = d2[k] * 0x1p40f, d = dt[k] * 0x1p40f;
+            x = fmaf(a, b, -d * d);
+            y = d * (a + b);
+          }
+          sacc += atan2_fast(y, x);
           corr += (s12[k] < 0.f && dt[k] != 0.f) ? (dt[k] > 0.f ? 1 : -1) : 0;
         }
       }

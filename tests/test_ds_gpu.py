@@ -173,3 +173,22 @@ def test_graph_replays_match_eager(engine):
     again = engine.linking_numbers(pairs, method="directsum", fallback_tol=0.0).lk
     assert np.array_equal(again, fresh)
     assert not np.array_equal(fresh, eager)
+
+
+def test_tiny_linked_rings_beside_a_large_curve(engine):
+    """The coordinate scale follows the largest curve (here a ring of radius
+    500), so a Hopf pair of radius 1e-4 has relative vectors near 1e-9 of the
+    unit: the products in the FP32 angle underflow and the kernel's rare path
+    redoes them from factors scaled by 2^40.  The FP32 sum must still match
+    the FP64 kernel and the oracle."""
+    c = W.workload(1, scale=0.2)
+    t = np.linspace(0, 2 * np.pi, 4000, endpoint=False)
+    big = np.stack([2000 + 500 * np.cos(t), 500 * np.sin(t), 0 * t], 1)
+    tiny = [1e-4 * c[0], 1e-4 * c[1]]
+    engine.set_curves(tiny + [big])
+    r32 = engine.linking_numbers([[0, 1]], method="directsum", fallback_tol=0.0)
+    r64 = engine.linking_numbers([[0, 1]], method="directsum", fp64=1)
+    ref = O.ds(tiny[0], tiny[1])
+    print("fp32", r32.lk[0], "fp64", r64.lk[0], "oracle", ref)
+    assert abs(r64.lk[0] - ref) < 1e-9 and abs(abs(ref) - 1) < 1e-6
+    assert abs(r32.lk[0] - ref) < 1e-5
