@@ -1,13 +1,15 @@
 """Training pipeline and weight files of the learned opening criterion.
 
-The network (12-64-64-1, SiLU) predicts log10 |exact - far field| of a
-Barnes–Hut node pair from 12 dimensionless features; samples and labels come
-from the engine itself (flk_mlp_samples: exact FP64 direct sum over the two
-nodes' segments against the FP32 second-order expansion), training runs on the
-GPU with the engine's bf16-MFMA forward/backward kernels and Adam.
+The network (12-64-64-1, hard-swish h(z) = z min(max(z/6 + 1/2, 0), 1))
+predicts log10 |exact - far field| of a Barnes–Hut node pair from 12
+dimensionless features; samples and labels come from the engine itself
+(flk_mlp_samples: exact FP64 direct sum over the two nodes' segments against
+the FP32 second-order expansion), training runs on the GPU with the engine's
+bf16-MFMA forward/backward kernels and Adam.
 
-Weight file: b"FLKMLP1\\0", 5,057 float32 parameters, 24 float32 feature
-normalisation values (mean[12], std[12]), little endian.
+Weight file: b"FLKMLP2\\0", 5,057 float32 parameters, 24 float32 feature
+normalisation values (mean[12], std[12]), little endian.  (b"FLKMLP1\\0"
+files hold weights of an earlier SiLU network and are refused.)
 """
 from __future__ import annotations
 
@@ -19,7 +21,7 @@ import numpy as np
 from . import MLP_NFEAT, MLP_NPARAM, Engine
 from . import workloads
 
-MAGIC = b"FLKMLP1\0"
+MAGIC = b"FLKMLP2\0"
 DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "mlp_criterion.bin")
 
 
@@ -36,6 +38,9 @@ def save_weights(path: str, params: np.ndarray, norm: np.ndarray) -> None:
 def load_weights(path: str = DEFAULT_WEIGHTS) -> tuple[np.ndarray, np.ndarray]:
     with open(path, "rb") as f:
         data = f.read()
+    if data[:8] == b"FLKMLP1\0":
+        raise ValueError(f"{path}: weights of the earlier SiLU network; retrain "
+                         "(python -m fastlinkingnumbers_amd.train_criterion)")
     if data[:8] != MAGIC or len(data) != 8 + 4 * (MLP_NPARAM + 2 * MLP_NFEAT):
         raise ValueError(f"{path}: not an MLP weight file")
     arr = np.frombuffer(data[8:], dtype="<f4")

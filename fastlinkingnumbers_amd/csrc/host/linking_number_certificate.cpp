@@ -44,8 +44,9 @@ void engine_check(flk_ctx *ctx, int32_t rc, const char *what) {
 // ---------------------------------------------------------------- helpers
 namespace {
 
-// Learned-criterion weight file: "FLKMLP1\0", 5,057 + 24 float32 (little
-// endian).  The default lives beside the package's lib/ directory.
+// Learned-criterion weight file: "FLKMLP2\0", 5,057 + 24 float32 (little
+// endian) of the hard-swish network.  The default lives beside the package's
+// lib/ directory.  "FLKMLP1\0" files hold an earlier SiLU network's weights.
 void load_mlp_weights(flk_ctx *ctx, std::string path) {
   if (path.empty()) {
     Dl_info info;
@@ -62,7 +63,9 @@ void load_mlp_weights(flk_ctx *ctx, std::string path) {
   std::vector<float> buf(FLK_MLP_NPARAM + 2 * FLK_MLP_NFEAT);
   f.read(magic, 8);
   f.read(reinterpret_cast<char *>(buf.data()), sizeof(float) * buf.size());
-  if (!f || std::memcmp(magic, "FLKMLP1\0", 8) != 0)
+  if (f && std::memcmp(magic, "FLKMLP1\0", 8) == 0)
+    throw std::runtime_error("MLP weight file of the earlier SiLU network, retrain: " + path);
+  if (!f || std::memcmp(magic, "FLKMLP2\0", 8) != 0)
     throw std::runtime_error("not an MLP weight file: " + path);
   engine_check(ctx, flk_mlp_set_weights(ctx, buf.data(), buf.data() + FLK_MLP_NPARAM),
                "flk_mlp_set_weights");

@@ -1,12 +1,19 @@
-// Learned opening criterion: a 12-64-64-1 SiLU MLP on bf16 matrix cores.
+// Learned opening criterion: a 12-64-64-1 hard-swish MLP on matrix cores.
 //
 // The network predicts log10 of the far-field error of a node pair from 12
 // dimensionless features (see mlp_features in bh.hip).  Parameters are FP32
-// masters (W1[64x12] b1[64] W2[64x64] b2[64] w3[64] b3, 5,057 floats) with
-// bf16 copies for v_mfma_f32_16x16x32_bf16: inputs are normalised by the
-// training mean/std, padded from 12 to 32 features, and every matrix product
-// of the forward and backward pass runs on MFMA tiles of 16 rows with FP32
-// accumulation; biases, SiLU and the output layer are FP32.
+// masters (W1[64x12] b1[64] W2[64x64] b2[64] w3[64] b3, 5,057 floats).  The
+// activation is hard-swish, h(z) = z min(max(z/6 + 1/2, 0), 1): the
+// logistic of SiLU costs two quarter-rate transcendental instructions per
+// activation (2^x and a reciprocal), which bounded inference, while
+// hard-swish is one packed FP16 FMA with the clamp bit and one packed
+// multiply per two activations.
+//
+// Training: inputs normalised by the training mean/std, padded from 12 to 32
+// features; every matrix product of the forward and backward pass runs on
+// v_mfma_f32_16x16x32_bf16 tiles of 16 rows with FP32 accumulation, the
+// biases, activations and output layer in FP32.  Inference (below) runs on
+// FP16 operands, v_mfma_f32_16x16x32_f16.
 //
 // MFMA 16x16x32 fragments (one wave): lane l supplies A[l%16][8*(l/16)+j] and
 // B[8*(l/16)+j][l%16] for j = 0..7 and receives C[4*(l/16)+i][l%16], i=0..3.
@@ -31,10 +38,14 @@ constexpr int H = 64;       // hidden width
 constexpr int KP = 32;      // padded input width
 constexpr int OFF_W1 = 0, OFF_B1 = 768, OFF_W2 = 832, OFF_B2 = 4928, OFF_W3 = 4992, OFF_B3 = 5056;
 
-// Logistic function with the hardware reciprocal (1 ulp; the activations are
-// rounded to bf16 or summed into an FP32 output): a full IEEE division would
-// cost ten instructions per activation.
-__device__ __forceinline__ float sigm(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+// Hard-swish in FP32 (training) and its derivative: 1/2 + z/3 on (-3, 3),
+// 0 at and below -3, 1 at and above 3.
+__device__ __forceinline__ float hswish(float z) {
+  return z * fminf(fmaxf(fmaf(z, 1.0f / 6.0f, 0.5f), 0.f), 1.f);
+}
+__device__ __forceinline__ float dhswish(float z) {
+  return z <= -3.f ? 0.f : (z < 3.f ? fmaf(z, 1.0f / 3.0f, 0.5f) : 1.f);
+}
 
 struct WaveLds {          // per-wave staging, bf16 row-major tiles of 16 rows
   __bf16 x[16][KP];
@@ -96,7 +107,7 @@ __device__ void forward16(const BlockLds &L, WaveLds &W, f32x4 (&z1)[4], f32x4 (
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       acc[i] += L.b1[nb * 16 + l % 16];
-      W.h1[4 * (l / 16) + i][nb * 16 + l % 16] = (__bf16)(acc[i] * sigm(acc[i]));
+      W.h1[4 * (l / 16) + i][nb * 16 + l % 16] = (__bf16)hswish(acc[i]);
     }
     z1[nb] = acc;
   }
@@ -119,7 +130,7 @@ __device__ void forward16(const BlockLds &L, WaveLds &W, f32x4 (&z1)[4], f32x4 (
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       acc[i] += L.b2[nb * 16 + l % 16];
-      part[i] += w3 * (acc[i] * sigm(acc[i]));
+      part[i] += w3 * hswish(acc[i]);
     }
     z2[nb] = acc;
   }
@@ -157,31 +168,43 @@ __device__ void stage_x(const BlockLds &L, WaveLds &W, const float *__restrict__
 // layer 1 holding neurons nb*16 + 4*(l/16) + i of sample l%16 -- exactly the
 // B fragment of layer 2 once W2's reduction index is permuted to the same
 // order (k-slot 8g + j of step s <-> neuron (2s + j/4)*16 + 4g + j%4).
-// The A fragments of both layers live in VGPRs for the whole kernel, the
-// biases, output weights and input normalisation in LDS; activations never
-// leave registers.  One tile = 16 samples = 12 MFMAs per wave.
+// The A fragments of both layers and this lane's output weights live in
+// VGPRs for the whole kernel, the biases and input normalisation in LDS;
+// activations never leave registers.  One tile = 16 samples = 12 MFMAs per
+// wave.
 //
-// Base-2 form: with z' = z log2 e, SiLU(z) = ln 2 * z' / (1 + 2^-z'), so the
-// layers run on W1' = W1 log2 e, b1' = b1 log2 e (giving z1'), h' = z' /
-// (1 + 2^-z') (= h / ln 2), then W2 h' + b2 log2 e = z2' and y = (w3 ln 2) h2'
-// + b3: an activation is one hardware 2^x, an add, a reciprocal and a
-// multiply, and the biases enter as the MFMA accumulator's initial value.
+// FP16 form: inputs, weights and activations are FP16 (v_mfma_f32_16x16x32_f16,
+// FP32 accumulation, the biases entering as the accumulator's initial
+// value).  A pair of layer outputs is rounded to FP16 (v_cvt_pk_f16_f32, to
+// nearest even), its gate min(max(z/6 + 1/2, 0), 1) is one v_pk_fma_f16 with
+// the clamp bit, the activation one v_pk_mul_f16, and the output layer one
+// v_dot2c_f32_f16 per pair (FP32 sum): 2 instructions per activation of
+// layer 1 and 2.5 of layer 2.
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 constexpr float kLog2e = 1.44269504088896340736f;
 constexpr float kLn2 = 0.69314718055994530942f;
 struct InferRegs {
-  bf16x8 a1[4], a2[4][2];
+  f16x8 a1[4], a2[4][2];
+  f16x2 w3[4][2];  // output weights of neurons nb*16 + 4g + {0, 1} and {2, 3}
 };
 struct InferLds {
-  float b1[H], b2[H], w3[H];  // b1 log2 e, b2 log2 e, w3 ln 2
+  float b1[H], b2[H];
   float mean[16], istd[16];    // features 12..15 pad: mean 0, istd 0
   float mean2[16], istd2[16];  // the same for log2 features
   float b3;
 };
 
-// h' = z' / (1 + 2^-z') = SiLU(z' ln 2) / ln 2 (hardware 2^x and reciprocal:
-// the activations are rounded to bf16 or summed into an FP32 output).
-__device__ __forceinline__ float silu2(float z) {
-  return z * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-z));
+// Hard-swish of two layer outputs in FP16 (one rounding per operation).
+__device__ __forceinline__ f16x2 hswish2(float a, float b) {
+  const f16x2 z = __builtin_convertvector(f32x2{a, b}, f16x2);
+  const f16x2 sixth = {(_Float16)(1.0f / 6.0f), (_Float16)(1.0f / 6.0f)};
+  const f16x2 half = {(_Float16)0.5f, (_Float16)0.5f};
+  const f16x2 gate = __builtin_elementwise_min(
+      __builtin_elementwise_max(__builtin_elementwise_fma(z, sixth, half), f16x2{0, 0}), f16x2{1, 1});
+  return z * gate;
 }
 
 __device__ void load_infer(const float *__restrict__ P, const float *__restrict__ norm, InferRegs &R,
@@ -192,20 +215,23 @@ __device__ void load_infer(const float *__restrict__ P, const float *__restrict_
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const int f = 8 * g + j;
-      R.a1[nb][j] = (__bf16)(f < FLK_MLP_NFEAT ? kLog2e * P[OFF_W1 + (nb * 16 + c) * FLK_MLP_NFEAT + f] : 0.0f);
+      R.a1[nb][j] = (_Float16)(f < FLK_MLP_NFEAT ? P[OFF_W1 + (nb * 16 + c) * FLK_MLP_NFEAT + f] : 0.0f);
     }
 #pragma unroll
     for (int st = 0; st < 2; ++st)
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
         const int neuron = (2 * st + j / 4) * 16 + 4 * g + (j & 3);
-        R.a2[nb][st][j] = (__bf16)P[OFF_W2 + (nb * 16 + c) * H + neuron];
+        R.a2[nb][st][j] = (_Float16)P[OFF_W2 + (nb * 16 + c) * H + neuron];
       }
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+      R.w3[nb][q] = f16x2{(_Float16)P[OFF_W3 + nb * 16 + 4 * g + 2 * q],
+                          (_Float16)P[OFF_W3 + nb * 16 + 4 * g + 2 * q + 1]};
   }
   for (int t = threadIdx.x; t < H; t += blockDim.x) {
-    L.b1[t] = kLog2e * P[OFF_B1 + t];
-    L.b2[t] = kLog2e * P[OFF_B2 + t];
-    L.w3[t] = kLn2 * P[OFF_W3 + t];
+    L.b1[t] = P[OFF_B1 + t];
+    L.b2[t] = P[OFF_B2 + t];
   }
   for (int t = threadIdx.x; t < 16; t += blockDim.x) {
     L.mean[t] = t < FLK_MLP_NFEAT ? norm[t] : 0.f;
@@ -219,28 +245,33 @@ __device__ void load_infer(const float *__restrict__ P, const float *__restrict_
 }
 
 // One tile: bx is this lane's B fragment of the normalised inputs (features
-// 8g..8g+7 of sample l%16, bf16).  Returns the prediction of sample l%16 in
+// 8g..8g+7 of sample l%16, FP16).  Returns the prediction of sample l%16 in
 // every lane.
-__device__ __forceinline__ float infer_tile(const InferRegs &R, const InferLds &L, bf16x8 bx) {
+__device__ __forceinline__ float infer_tile(const InferRegs &R, const InferLds &L, f16x8 bx) {
   const int l = threadIdx.x & 63, g = l >> 4;
-  bf16x8 bh[2];
+  f16x2 h[4][2];
 #pragma unroll
   for (int nb = 0; nb < 4; ++nb) {
     const float4 bb = *reinterpret_cast<const float4 *>(&L.b1[nb * 16 + 4 * g]);
-    const f32x4 z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.a1[nb], bx, f32x4{bb.x, bb.y, bb.z, bb.w}, 0, 0, 0);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) bh[nb >> 1][(nb & 1) * 4 + i] = (__bf16)silu2(z[i]);
+    const f32x4 z = __builtin_amdgcn_mfma_f32_16x16x32_f16(R.a1[nb], bx, f32x4{bb.x, bb.y, bb.z, bb.w}, 0, 0, 0);
+    h[nb][0] = hswish2(z[0], z[1]);
+    h[nb][1] = hswish2(z[2], z[3]);
   }
+  // layer 2's B fragments: steps 0 and 1 hold the outputs of blocks 0, 1 and 2, 3
+  const f16x8 bh0 = __builtin_shufflevector(__builtin_shufflevector(h[0][0], h[0][1], 0, 1, 2, 3),
+                                            __builtin_shufflevector(h[1][0], h[1][1], 0, 1, 2, 3), 0, 1, 2,
+                                            3, 4, 5, 6, 7);
+  const f16x8 bh1 = __builtin_shufflevector(__builtin_shufflevector(h[2][0], h[2][1], 0, 1, 2, 3),
+                                            __builtin_shufflevector(h[3][0], h[3][1], 0, 1, 2, 3), 0, 1, 2,
+                                            3, 4, 5, 6, 7);
   float part = 0.f;
 #pragma unroll
   for (int nb = 0; nb < 4; ++nb) {
     const float4 bb = *reinterpret_cast<const float4 *>(&L.b2[nb * 16 + 4 * g]);
-    f32x4 z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.a2[nb][0], bh[0], f32x4{bb.x, bb.y, bb.z, bb.w}, 0, 0, 0);
-    z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.a2[nb][1], bh[1], z, 0, 0, 0);
-    const float4 ww = *reinterpret_cast<const float4 *>(&L.w3[nb * 16 + 4 * g]);
-    const float wv[4] = {ww.x, ww.y, ww.z, ww.w};
-#pragma unroll
-    for (int i = 0; i < 4; ++i) part = fmaf(wv[i], silu2(z[i]), part);
+    f32x4 z = __builtin_amdgcn_mfma_f32_16x16x32_f16(R.a2[nb][0], bh0, f32x4{bb.x, bb.y, bb.z, bb.w}, 0, 0, 0);
+    z = __builtin_amdgcn_mfma_f32_16x16x32_f16(R.a2[nb][1], bh1, z, 0, 0, 0);
+    part = __builtin_amdgcn_fdot2(hswish2(z[0], z[1]), R.w3[nb][0], part, false);
+    part = __builtin_amdgcn_fdot2(hswish2(z[2], z[3]), R.w3[nb][1], part, false);
   }
   part += __shfl_xor(part, 16, 64);
   part += __shfl_xor(part, 32, 64);
@@ -259,13 +290,13 @@ __global__ __launch_bounds__(256) void mlp_forward_kernel(const float *__restric
   const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
   for (int64_t r0 = wave * 16; r0 < n; r0 += nwaves * 16) {
     const int64_t row = r0 + c;
-    bf16x8 bx;
+    f16x8 bx;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const int f = 8 * g + j;
       const float v =
           (row < n && f < FLK_MLP_NFEAT) ? (x[row * FLK_MLP_NFEAT + f] - L.mean[f]) * L.istd[f] : 0.f;
-      bx[j] = (__bf16)v;
+      bx[j] = (_Float16)v;
     }
     const float yv = infer_tile(R, L, bx);
     if (g == 0 && row < n) y[row] = yv;
@@ -287,7 +318,7 @@ __global__ __launch_bounds__(256) void bh_mlp_predict(const BhItem *__restrict__
                                                       float *__restrict__ pred,
                                                       unsigned long long *__restrict__ evals) {
   __shared__ InferLds L;
-  __shared__ __attribute__((aligned(16))) __bf16 xs[4][64][16];
+  __shared__ __attribute__((aligned(16))) _Float16 xs[4][64][16];
   __shared__ int32_t src[4][64];
   __shared__ unsigned int block_evals;  // network evaluations of the block
   if (n_dev) n = level_count(n_dev, n);     // level enqueued ahead: n is the bound
@@ -302,6 +333,9 @@ __global__ __launch_bounds__(256) void bh_mlp_predict(const BhItem *__restrict__
   int64_t t = wave * 64 + l;
   BhItem it = t < n ? items[t] : BhItem{0, 0, 0, 0};
   BhGeo A = geo[it.a], B = geo[it.b];
+  // items run one iteration ahead of the records, so the record loads issued
+  // before the tiles never wait for the item load that addresses them
+  if (t + stride < n) it = items[t + stride];
   for (int64_t base = wave * 64; base < n; base += stride, t += stride) {
     bool need = false;
     if (t < n) {
@@ -315,14 +349,14 @@ __global__ __launch_bounds__(256) void bh_mlp_predict(const BhItem *__restrict__
         mlp_features_t<true>(A, B, dsq, f);  // log2 features, normalised with mean2 / istd2
         const unsigned long long m = __ballot(true);  // lanes with need == true
         const int pos = __popcll(m & ((1ull << l) - 1ull));
-        bf16x8 v0, v1;
+        f16x8 v0, v1;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) v0[j] = (__bf16)((f[j] - L.mean2[j]) * L.istd2[j]);
+        for (int j = 0; j < 8; ++j) v0[j] = (_Float16)((f[j] - L.mean2[j]) * L.istd2[j]);
 #pragma unroll
         for (int j = 0; j < 8; ++j)
-          v1[j] = (__bf16)(j < FLK_MLP_NFEAT - 8 ? (f[8 + j] - L.mean2[8 + j]) * L.istd2[8 + j] : 0.f);
-        *reinterpret_cast<bf16x8 *>(&xs[w][pos][0]) = v0;
-        *reinterpret_cast<bf16x8 *>(&xs[w][pos][8]) = v1;
+          v1[j] = (_Float16)(j < FLK_MLP_NFEAT - 8 ? (f[8 + j] - L.mean2[8 + j]) * L.istd2[8 + j] : 0.f);
+        *reinterpret_cast<f16x8 *>(&xs[w][pos][0]) = v0;
+        *reinterpret_cast<f16x8 *>(&xs[w][pos][8]) = v1;
         src[w][pos] = l;
       } else {
         pred[t] = __builtin_inff();
@@ -332,20 +366,21 @@ __global__ __launch_bounds__(256) void bh_mlp_predict(const BhItem *__restrict__
     wave_evals += (unsigned int)cnt;
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    // next iteration's records, in flight while the tiles run
+    // next iteration's records and the item after it, in flight while the
+    // tiles run
     const int64_t tn = t + stride;
     if (tn < n) {
-      it = items[tn];
       A = geo[it.a];
       B = geo[it.b];
+      if (tn + stride < n) it = items[tn + stride];
     }
     for (int r0 = 0; r0 < cnt; r0 += 16) {
       const int row = r0 + c;
-      bf16x8 bx;
-      if (g < 2 && row < cnt) bx = *reinterpret_cast<const bf16x8 *>(&xs[w][row][8 * g]);
+      f16x8 bx;
+      if (g < 2 && row < cnt) bx = *reinterpret_cast<const f16x8 *>(&xs[w][row][8 * g]);
       else
 #pragma unroll
-        for (int j = 0; j < 8; ++j) bx[j] = (__bf16)0.f;
+        for (int j = 0; j < 8; ++j) bx[j] = (_Float16)0.f;
       const float yv = infer_tile(R, L, bx);
       if (g == 0 && row < cnt) pred[base + src[w][row]] = yv;
     }
@@ -397,7 +432,7 @@ __global__ __launch_bounds__(256) void mlp_grad_kernel(const float *__restrict__
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    // dZ2 = dy w3 * silu'(z2); dw3 += dy * h2; db2 += dZ2
+    // dZ2 = dy w3 * h'(z2); dw3 += dy * h2; db2 += dZ2
 #pragma unroll
     for (int nb = 0; nb < 4; ++nb) {
       const int col = nb * 16 + l % 16;
@@ -406,16 +441,16 @@ __global__ __launch_bounds__(256) void mlp_grad_kernel(const float *__restrict__
       for (int i = 0; i < 4; ++i) {
         const int row = 4 * (l / 16) + i;
         const float dy = W.dy[row];
-        const float z = z2[nb][i], s = sigm(z);
-        gw3[nb] += dy * z * s;
-        const float d = dy * w3 * s * (1.0f + z * (1.0f - s));
+        const float z = z2[nb][i];
+        gw3[nb] += dy * hswish(z);
+        const float d = dy * w3 * dhswish(z);
         gb2[nb] += d;
         W.dz2[row][col] = (__bf16)d;
       }
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    // dH1 = dZ2 W2 ; dZ1 = dH1 * silu'(z1)
+    // dH1 = dZ2 W2 ; dZ1 = dH1 * h'(z1)
     {
       const bf16x8 a0 = a_frag(&W.dz2[0][0], H, 0), a1 = a_frag(&W.dz2[0][0], H, 32);
 #pragma unroll
@@ -431,8 +466,7 @@ __global__ __launch_bounds__(256) void mlp_grad_kernel(const float *__restrict__
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, acc, 0, 0, 0);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          const float z = z1[nb][i], s = sigm(z);
-          const float d = acc[i] * s * (1.0f + z * (1.0f - s));
+          const float d = acc[i] * dhswish(z1[nb][i]);
           gb1v[nb] += d;
           W.dz1[4 * (l / 16) + i][nb * 16 + l % 16] = (__bf16)d;
         }

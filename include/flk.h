@@ -297,14 +297,15 @@ int32_t flk_autotune(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, const f
 
 /* Learned opening criterion (no reference counterpart; it replaces the beta
  * test of src/compute_link.cpp:573-574 and :608).  Weights are FP32 masters of
- * the 12-64-64-1 SiLU MLP: W1[64x12] b1[64] W2[64x64] b2[64] W3[64] b3[1]
- * (5,057 floats), plus 24 floats of feature normalisation (mean[12], std[12]).
+ * the 12-64-64-1 MLP with hard-swish activations, h(z) = z min(max(z/6 + 1/2,
+ * 0), 1): W1[64x12] b1[64] W2[64x64] b2[64] W3[64] b3[1] (5,057 floats), plus
+ * 24 floats of feature normalisation (mean[12], std[12]).
  */
 #define FLK_MLP_NPARAM 5057
 #define FLK_MLP_NFEAT 12
 int32_t flk_mlp_set_weights(flk_ctx *ctx, const float *params, const float *norm24);
 int32_t flk_mlp_get_weights(const flk_ctx *ctx, float *params, float *norm24);
-/* Forward pass of the bf16-MFMA inference kernel: x is n x 12 FP32 features
+/* Forward pass of the FP16-MFMA inference kernel: x is n x 12 FP32 features
  * (raw, normalised inside), y receives n predicted log10 errors. */
 int32_t flk_mlp_forward(flk_ctx *ctx, const float *x, int64_t n, float *y);
 /* One full-batch gradient of the mean squared error over (x, y) with the
