@@ -132,9 +132,7 @@ __global__ __launch_bounds__(256) void bh_split(const BhRange *__restrict__ lev,
   }
 }
 
-__host__ __device__ __forceinline__ int qidx(int i, int j) {  // symmetric 3x3 -> 6
-  return i <= j ? (i == 0 ? j : (i == 1 ? 2 + j : 5)) : (j == 0 ? i : (j == 1 ? 2 + i : 5));
-}
+__host__ __device__ __forceinline__ int qidx(int i, int j) { return sym_idx(i, j); }
 
 // Endpoints of every segment in sorted order, gathered once after the sort
 // (six FP64 values per segment, copied unchanged from the vertex array), so
@@ -269,6 +267,12 @@ __device__ __forceinline__ void direct_records(BhRange rr, int2 ch, const double
   gr.dn = sqrtf(dn) * 0.57735026918962576451f;
   gr.qn = (1.0f / 3.0f) * sqrtf(qn);
   gr.pad = 0.f;
+  if (gr.nseg == 1) {  // a one-segment leaf: its tangent instead of a BhMom record
+    gr.c0 = kLeafSegment;
+    gr.c1 = __builtin_bit_cast(int32_t, m.c[0]);
+    gr.pad = m.c[1];
+    gr.dn = m.c[2];
+  }
 }
 
 __device__ __forceinline__ void wave_lds_sync() {
@@ -279,7 +283,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 // Stores the FP32 records of a wave's nw consecutive nodes (lane l holds
 // node l's; geo and mom point at the wave's first node) through the wave's
 // LDS buffer sw (at least 256 float4), so each store instruction writes 1 KB
-// of contiguous memory.
+// of contiguous memory.  One-segment leaves write no BhMom record.
 __device__ __forceinline__ void wave_store_records(const BhGeo &gr, const BhMom &m, BhGeo *__restrict__ geo,
                                                    BhMom *__restrict__ mom, int nw, float4 *sw) {
   const int lane = threadIdx.x & 63;
@@ -290,16 +294,19 @@ __device__ __forceinline__ void wave_store_records(const BhGeo &gr, const BhMom 
   float4 *gdst = reinterpret_cast<float4 *>(geo);
   for (int c = lane; c < 4 * nw; c += 64) gdst[c] = sw[c];
   wave_lds_sync();
+  const unsigned long long keep = __ballot(lane < nw && !is_leaf_segment(gr));
   const float4 *m4 = reinterpret_cast<const float4 *>(&m);
   float4 *mdst = reinterpret_cast<float4 *>(mom);
 #pragma unroll
   for (int h = 0; h < 2; ++h) {  // records of lanes 32h .. 32h + 31
+    if (((keep >> (32 * h)) & 0xffffffffull) == 0) continue;  // wave-uniform
     if ((lane >> 5) == h)
 #pragma unroll
       for (int i = 0; i < 8; ++i) sw[8 * (lane & 31) + i] = m4[i];
     wave_lds_sync();
     const int lim = 8 * min(32, nw - 32 * h);
-    for (int c = lane; c < lim; c += 64) mdst[256 * h + c] = sw[c];
+    for (int c = lane; c < lim; c += 64)
+      if ((keep >> (32 * h + c / 8)) & 1ull) mdst[256 * h + c] = sw[c];
     wave_lds_sync();
   }
 }
@@ -504,7 +511,8 @@ __device__ float far_field(const BhMom &A, const BhMom &B, const BhGeo &GA, cons
                    2.0f * dot3(Dv(A.D, r), cross3(r, Dv(B.D, r)));
   acc -= 1.5f * ir2 * ((s2a + s2b + 2.0f * s3) + 5.0f * ir2 * s1);
   const float mult = ir2 * __builtin_amdgcn_rsqf(dsq) * kInv4PiF;
-  err = mult * ir2 * (GA.qn * (GA.radius * GB.cn + 3.0f * GB.dn) + GB.qn * (GB.radius * GA.cn + 3.0f * GA.dn));
+  err = mult * ir2 *
+        (GA.qn * (GA.radius * GB.cn + 3.0f * geo_dn(GB)) + GB.qn * (GB.radius * GA.cn + 3.0f * geo_dn(GA)));
   return mult * acc;
 }
 
@@ -712,7 +720,7 @@ __global__ __launch_bounds__(kTravBlock) void bh_level(
       } else if (far) {
         kind = 1;
         float e = 0.f;
-        const float val = far_field(mom[it.a], mom[it.b], A, B, r, dsq, e);
+        const float val = far_field(node_mom(mom, it.a, A), node_mom(mom, it.b, B), A, B, r, dsq, e);
         if (pred) e = exp10f(2.0f * pred[t]);
         if (tp < 0 || tp == it.pair) {
           tp = it.pair;
@@ -929,7 +937,7 @@ __global__ __launch_bounds__(256) void bh_sample_labels(
                     (B.cz - A.cz) + (B.lz - A.lz));
     const float dsq = dot3(r, r) + eps;
     float e;
-    const float ff = far_field(mom[it.a], mom[it.b], A, B, r, dsq, e);
+    const float ff = far_field(node_mom(mom, it.a, A), node_mom(mom, it.b, B), A, B, r, dsq, e);
     const double d = fabs(exact - (double)ff);
     y[w] = (float)fmax(log10(d), -14.0);
     float f[FLK_MLP_NFEAT];
@@ -1536,7 +1544,7 @@ void tree_export(flk_ctx *ctx, int32_t curve, int32_t leaf, std::vector<flk_tree
   nodes.resize(nn);
   for (int32_t k = 0; k < nn; ++k) {
     const BhGeo &a = g[k];
-    const BhMom &b = m[k];
+    const BhMom b = is_leaf_segment(a) ? leaf_segment_mom(a) : m[k];
     flk_tree_node &o = nodes[k];
     std::memset(&o, 0, sizeof o);
     const double ctr[3] = {(double)a.cx + (double)a.lx, (double)a.cy + (double)a.ly, (double)a.cz + (double)a.lz};
@@ -1549,7 +1557,7 @@ void tree_export(flk_ctx *ctx, int32_t curve, int32_t leaf, std::vector<flk_tree
       for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) o.Q[9 * kk + 3 * i + j] = (double)b.Q[6 * kk + qidx(i, j)] * is3;
     o.cn = (double)a.cn * is;
-    o.dn = (double)a.dn * is2;
+    o.dn = (double)geo_dn(a) * is2;
     o.qn = (double)a.qn * is3;
     o.child[0] = a.c0 < 0 ? -1 : a.c0;
     o.child[1] = a.c0 < 0 ? -1 : a.c1;

@@ -23,6 +23,40 @@ struct BhItem {
   int32_t pair, a, b, pad;
 };
 
+// A leaf of one segment stores no BhMom record (half the nodes of a tree of
+// one-segment leaves; their records were a third of the bytes the tree build
+// writes).  Its moments about its own centre, the segment's midpoint, are
+// c = t, D = 0 and Q_k = t_k t t^T / 12 with t the segment's FP32 tangent,
+// so its geometry record carries t in three slots such a leaf does not
+// otherwise use -- c1 (the bits of t.x), pad (t.y) and dn (t.z; its true dn
+// is 0) -- and c0 = kLeafSegment marks it (c0 < 0 still means "leaf").
+constexpr int32_t kLeafSegment = -2;
+__host__ __device__ __forceinline__ bool is_leaf_segment(const BhGeo &g) { return g.c0 == kLeafSegment; }
+__host__ __device__ __forceinline__ float geo_dn(const BhGeo &g) { return is_leaf_segment(g) ? 0.f : g.dn; }
+__host__ __device__ __forceinline__ int sym_idx(int i, int j) {  // symmetric 3x3 -> 6: xx xy xz yy yz zz
+  return i <= j ? (i == 0 ? j : (i == 1 ? 2 + j : 5)) : (j == 0 ? i : (j == 1 ? 2 + i : 5));
+}
+// The moment record of a one-segment leaf, bit for bit what summing its
+// single segment gives (the midpoint offset is exactly zero).
+__host__ __device__ __forceinline__ BhMom leaf_segment_mom(const BhGeo &g) {
+  float t[3];
+  t[0] = __builtin_bit_cast(float, g.c1);
+  t[1] = g.pad;
+  t[2] = g.dn;
+  BhMom m;
+  for (int i = 0; i < 3; ++i) m.c[i] = t[i];
+  for (int k = 0; k < 9; ++k) m.D[k] = 0.f;
+  for (int k = 0; k < 3; ++k)
+    for (int i = 0; i < 3; ++i)
+      for (int j = i; j < 3; ++j)
+        m.Q[6 * k + sym_idx(i, j)] = fmaf(t[k], fmaf((1.0f / 12.0f) * t[i], t[j], 0.f), 0.f);
+  m.pad[0] = m.pad[1] = 0.f;
+  return m;
+}
+__device__ __forceinline__ BhMom node_mom(const BhMom *__restrict__ mom, int32_t id, const BhGeo &g) {
+  return is_leaf_segment(g) ? leaf_segment_mom(g) : mom[id];
+}
+
 // Items of a level whose count was produced on the device, for a kernel whose
 // buffer holds `bound` items: a count above the bound means the producer
 // overflowed the buffer (its last slot may be unwritten), and the level is
@@ -49,19 +83,20 @@ __device__ __forceinline__ void mlp_features_t(const BhGeo &A, const BhGeo &B, f
   auto lg = [](float x) { return kLog2 ? __builtin_amdgcn_logf(x) : ln_normal(x); };
   const float id = __builtin_amdgcn_rsqf(dsq), id2 = id * id;
   const float tiny = 1e-12f;
+  const float adn = geo_dn(A), bdn = geo_dn(B);
   f[0] = lg(A.radius * id + tiny);
   f[1] = lg(B.radius * id + tiny);
   f[2] = lg((A.radius + B.radius) * id + tiny);
   f[3] = lg(A.cn * id + tiny);
-  f[4] = lg(A.dn * id2 + tiny);
+  f[4] = lg(adn * id2 + tiny);
   f[5] = lg(A.qn * id2 * id + tiny);
   f[6] = lg(B.cn * id + tiny);
-  f[7] = lg(B.dn * id2 + tiny);
+  f[7] = lg(bdn * id2 + tiny);
   f[8] = lg(B.qn * id2 * id + tiny);
   f[9] = lg((float)A.nseg);
   f[10] = lg((float)B.nseg);
   const float est = 0.0795774715f * id2 * id2 * id *
-                    (A.qn * (A.radius * B.cn + 3.0f * B.dn) + B.qn * (B.radius * A.cn + 3.0f * A.dn));
+                    (A.qn * (A.radius * B.cn + 3.0f * bdn) + B.qn * (B.radius * A.cn + 3.0f * adn));
   f[11] = lg(est + tiny);
 }
 __device__ __forceinline__ void mlp_features(const BhGeo &A, const BhGeo &B, float dsq, float *f) {
