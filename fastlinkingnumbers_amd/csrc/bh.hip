@@ -38,6 +38,7 @@
 #include <type_traits>
 
 #include "bh_types.h"
+#include "mlp_infer.h"
 #include "flk_internal.h"
 
 namespace flk {
@@ -663,16 +664,72 @@ __device__ __forceinline__ int wave_lds_slot(bool pred, int *counter) {
   return pred ? base + __popcll(below) : -1;
 }
 
+// Learned criterion inside the level kernel: the network's biases and input
+// normalisation, and each wave's rows of features and predictions.
+struct MlpShared {
+  InferLds L;
+  InferWLds W;  // the weight fragments: the level kernel has no VGPRs to spare for them
+  __attribute__((aligned(16))) _Float16 xs[kTravBlock / 64][64][16];
+  float ys[kTravBlock / 64][64];
+  unsigned int evals;  // network evaluations of the block
+};
+struct NoMlpShared {};
+
+// Predictions of the learned criterion for the wave's 64 items (one per
+// lane; log10 of the far-field error, +inf where the decision does not
+// depend on it): the features of the items that need the network are
+// compacted into the wave's LDS rows and evaluated 16 per tile, and each lane
+// reads its item's prediction back.  Wave-uniform: every lane calls it.
+__device__ __forceinline__ float wave_predict(bool need, const BhGeo &A, const BhGeo &B, float dsq, MlpShared &ms,
+                                              unsigned int &wave_evals) {
+  const unsigned long long m = __ballot(need);
+  if (m == 0) return __builtin_inff();
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6, g = l >> 4, c = l & 15;
+  const int pos = __popcll(m & ((1ull << l) - 1ull));
+  if (need) {
+    float f[FLK_MLP_NFEAT];
+    mlp_features_t<true>(A, B, dsq, f);  // log2 features, normalised with mean2 / istd2
+    f16x8 v0, v1;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v0[j] = (_Float16)((f[j] - ms.L.mean2[j]) * ms.L.istd2[j]);
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      v1[j] = (_Float16)(j < FLK_MLP_NFEAT - 8 ? (f[8 + j] - ms.L.mean2[8 + j]) * ms.L.istd2[8 + j] : 0.f);
+    *reinterpret_cast<f16x8 *>(&ms.xs[w][pos][0]) = v0;
+    *reinterpret_cast<f16x8 *>(&ms.xs[w][pos][8]) = v1;
+  }
+  wave_lds_sync();
+  const int cnt = __popcll(m);
+  wave_evals += (unsigned int)cnt;
+  for (int r0 = 0; r0 < cnt; r0 += 16) {
+    const int row = r0 + c;
+    f16x8 bx;
+    if (g < 2 && row < cnt) bx = *reinterpret_cast<const f16x8 *>(&ms.xs[w][row][8 * g]);
+    else
+#pragma unroll
+      for (int j = 0; j < 8; ++j) bx[j] = (_Float16)0.f;
+    const float yv = infer_tile(LdsWeights{(LdsW *)&ms.W, l}, ms.L, bx);
+    if (g == 0 && row < cnt) ms.ys[w][row] = yv;
+  }
+  wave_lds_sync();
+  const float p = need ? ms.ys[w][pos] : __builtin_inff();
+  wave_lds_sync();  // every lane has its prediction before the rows are reused
+  return p;
+}
+
 // One traversal level over a chunk of items.  Split items are staged in LDS
 // as (pair with the split side in its sign, unsplit node, child 0, child 1)
 // from the front of the block's stage, leaf-bucket items from the back; after
 // the block has classified its 256 * IPT items the stage is copied out
-// coalesced behind one queue atomic each.
-template <int IPT>
-__global__ __launch_bounds__(kTravBlock) void bh_level(
+// coalesced behind one queue atomic each.  MLP: the learned criterion, its
+// network evaluated here on the node records the level already holds (the
+// weights in VGPRs, loaded once per block); *evals counts the evaluations.
+template <int IPT, bool MLP>
+__global__ __launch_bounds__(kTravBlock) __attribute__((amdgpu_waves_per_eu(4))) void bh_level(
     const BhItem *__restrict__ items, int64_t n, const unsigned long long *__restrict__ n_dev,
     const BhGeo *__restrict__ geo, const BhMom *__restrict__ mom, const float *__restrict__ beta2,
-    float eps, const float *__restrict__ pred, float log_tol,
+    float eps, const float *__restrict__ mlp_params, const float *__restrict__ mlp_norm, float log_tol,
+    unsigned long long *__restrict__ evals,
     BhItem *__restrict__ next, unsigned long long *__restrict__ next_count, int64_t next_cap,
     BhItem *__restrict__ leaves, unsigned long long *__restrict__ leaf_count, int64_t leaf_cap,
     double *__restrict__ psum, double *__restrict__ perr) {
@@ -688,6 +745,18 @@ __global__ __launch_bounds__(kTravBlock) void bh_level(
   if (n_dev) n = level_count(n_dev, n);
   __shared__ int4 stage[kTravItems];
   __shared__ TravShared sh;
+  __shared__ std::conditional_t<MLP, MlpShared, NoMlpShared> ms;
+  unsigned int wave_evals = 0;
+  if constexpr (MLP) {
+    if ((int64_t)blockIdx.x * kTravItems >= n) return;  // the whole block, before any barrier
+    {
+      InferRegs R;
+      load_infer(mlp_params, mlp_norm, R, ms.L);  // ends with a block barrier
+      if (threadIdx.x < 64) infer_w_store(R, ms.W);
+    }
+    if (threadIdx.x == 0) ms.evals = 0;  // read after the chunk loop's barriers
+    __syncthreads();
+  }
   for (int64_t chunk = blockIdx.x; chunk * kTravItems < n; chunk += gridDim.x) {
   if (threadIdx.x == 0) sh.nsplit = sh.nleaf = 0;
   __syncthreads();
@@ -702,6 +771,15 @@ __global__ __launch_bounds__(kTravBlock) void bh_level(
     int kind = -1;  // 0 leaf pair, 1 far, 2 split
     int ca = 0, cb = 0;
     bool split_a = false;
+    float pv = __builtin_inff();  // learned criterion: predicted log10 far-field error
+    if constexpr (MLP) {
+      const BhGeo A = geo[it.a], B = geo[it.b];
+      const float rx = (B.cx - A.cx) + (B.lx - A.lx), ry = (B.cy - A.cy) + (B.ly - A.ly),
+                  rz = (B.cz - A.cz) + (B.lz - A.lz);
+      const float dsq = fmaf(rx, rx, fmaf(ry, ry, rz * rz)) + eps;
+      const float rs = A.radius + B.radius;
+      pv = wave_predict(live && !(A.c0 < 0 && B.c0 < 0) && dsq > rs * rs + eps, A, B, dsq, ms, wave_evals);
+    }
     if (live) {
       const BhGeo A = geo[it.a], B = geo[it.b];
       const F3 r = f3((B.cx - A.cx) + (B.lx - A.lx), (B.cy - A.cy) + (B.ly - A.ly),
@@ -713,15 +791,14 @@ __global__ __launch_bounds__(kTravBlock) void bh_level(
       // predicted log10 error is below the tolerance; the squared predicted
       // error is accumulated (the pair's estimate is the root of the sum, as
       // for independent signs).  Otherwise the reference's beta test.
-      const bool far = pred ? (dsq > rs * rs + eps && pred[t] < log_tol)
-                            : (dsq > rs * rs * beta2[it.pair] + eps);
+      const bool far = MLP ? (dsq > rs * rs + eps && pv < log_tol) : (dsq > rs * rs * beta2[it.pair] + eps);
       if (la && lb) {
         kind = 0;
       } else if (far) {
         kind = 1;
         float e = 0.f;
         const float val = far_field(node_mom(mom, it.a, A), node_mom(mom, it.b, B), A, B, r, dsq, e);
-        if (pred) e = exp10f(2.0f * pred[t]);
+        if (MLP) e = exp10f(2.0f * pv);
         if (tp < 0 || tp == it.pair) {
           tp = it.pair;
           tv += (double)val;
@@ -772,6 +849,11 @@ __global__ __launch_bounds__(kTravBlock) void bh_level(
     }
   }
   __syncthreads();  // the stage and the shared counters are reused by the next chunk
+  }
+  if constexpr (MLP) {
+    if ((threadIdx.x & 63) == 0 && wave_evals) atomicAdd(&ms.evals, wave_evals);
+    __syncthreads();
+    if (threadIdx.x == 0 && ms.evals) atomicAdd(evals, (unsigned long long)ms.evals);
   }
 }
 
@@ -1162,6 +1244,16 @@ void build_forest(flk_ctx *ctx, const std::vector<int32_t> &curves_in, int leaf,
   for (int32_t k = 0; k < nc; ++k) F.root[cs[k]] = k;  // roots are nodes 0..nc-1
 }
 
+// The level kernel for a criterion and a number of items per thread.
+using LevelKernel = void (*)(const BhItem *, int64_t, const unsigned long long *, const BhGeo *, const BhMom *,
+                             const float *, float, const float *, const float *, float, unsigned long long *,
+                             BhItem *, unsigned long long *, int64_t, BhItem *, unsigned long long *, int64_t,
+                             double *, double *);
+LevelKernel level_kernel(int ipt, bool use_mlp) {
+  if (use_mlp) return ipt == 2 ? bh_level<2, true> : ipt == 8 ? bh_level<8, true> : bh_level<4, true>;
+  return ipt == 2 ? bh_level<2, false> : ipt == 8 ? bh_level<8, false> : bh_level<4, false>;
+}
+
 // Traverses the given pairs (indices into the forest roots) with per-pair
 // beta; adds into psum/perr (device, per pair).
 //
@@ -1215,10 +1307,9 @@ void traverse(flk_ctx *ctx, Forest &F, const std::vector<int32_t> &ra, const std
                      B.roots_a.as<int32_t>(), B.roots_b.as<int32_t>(), np, stack[0].buf->as<BhItem>());
   FLK_HIP(hipGetLastError());
   DevBuf *spare = level_buf(2 * std::min(np, kChunk));
-  if (use_mlp) {
-    mlp_upload(ctx);
-    B.pred.reserve(sizeof(float) * std::min(np, kChunk));
-  }
+  if (use_mlp) mlp_upload(ctx);
+  const float *mlp_params = use_mlp ? ctx->mlp.d_params.as<float>() : nullptr;
+  const float *mlp_norm = use_mlp ? ctx->mlp.d_norm.as<float>() : nullptr;
   unsigned long long visits = 0, far = 0;
   int slot_next = 0;
   DevBuf *alt = nullptr;  // second output buffer of a multi-level round
@@ -1252,7 +1343,6 @@ void traverse(flk_ctx *ctx, Forest &F, const std::vector<int32_t> &ra, const std
       FLK_HIP(hipStreamSynchronize(s));  // the previous leaf launch may still read it
       B.leaves.reserve(sizeof(BhItem) * leaf_cap);
     }
-    if (use_mlp) B.pred.reserve(sizeof(float) * (take << (D - 1)));
     if (slot_next + D > kSlots) {
       FLK_HIP(hipMemsetAsync(cnt + 4, 0, sizeof(unsigned long long) * 2 * kSlots, s));
       slot_next = 0;
@@ -1265,25 +1355,21 @@ void traverse(flk_ctx *ctx, Forest &F, const std::vector<int32_t> &ra, const std
       const BhItem *in = d == 0 ? src : outb[d - 1]->as<BhItem>();
       const int64_t bound = take << d;
       const unsigned long long *n_dev = d == 0 ? nullptr : qc + 2 * (d - 1);
-      if (use_mlp)
-        mlp_predict_items(ctx, in, bound, n_dev, F.geo.as<BhGeo>(), eps, B.pred.as<float>(), cnt + 3,
-                          timing ? ctx->ev0 : nullptr, timing ? ctx->ev3 : nullptr);
       // with timing, start/stop events ride on the dispatch itself
       const dim3 grid((unsigned)((bound + ipt * kTravBlock - 1) / (ipt * kTravBlock)));
-      auto lev = ipt == 2 ? bh_level<2> : ipt == 8 ? bh_level<8> : bh_level<4>;
-      launch_timed(timing ? ctx->ev1 : nullptr, timing ? ctx->ev2 : nullptr, lev, grid, dim3(kTravBlock), s,
-                   in, bound, n_dev, F.geo.as<BhGeo>(), F.mom.as<BhMom>(), (const float *)B.beta2.as<float>(),
-                   eps, (const float *)(use_mlp ? B.pred.as<float>() : nullptr), log_tol, outb[d]->as<BhItem>(),
-                   qc + 2 * d, 2 * bound, B.leaves.as<BhItem>(), qc + 1, leaf_cap, d_sum, d_err);
+      launch_timed(timing ? ctx->ev1 : nullptr, timing ? ctx->ev2 : nullptr, level_kernel(ipt, use_mlp), grid,
+                   dim3(kTravBlock), s, in, bound, n_dev, F.geo.as<BhGeo>(), F.mom.as<BhMom>(),
+                   (const float *)B.beta2.as<float>(), eps, mlp_params, mlp_norm, log_tol, cnt + 3,
+                   outb[d]->as<BhItem>(), qc + 2 * d, 2 * bound, B.leaves.as<BhItem>(), qc + 1, leaf_cap, d_sum,
+                   d_err);
       FLK_HIP(hipGetLastError());
     }
     FLK_HIP(hipMemcpyAsync(B.host_cnt, qc, 2 * D * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     FLK_HIP(hipStreamSynchronize(s));
     if (timing) {
-      float ms_pred = 0.f, ms_level = 0.f;
-      if (use_mlp) FLK_HIP(hipEventElapsedTime(&ms_pred, ctx->ev0, ctx->ev3));
+      float ms_level = 0.f;
       FLK_HIP(hipEventElapsedTime(&ms_level, ctx->ev1, ctx->ev2));
-      ctx->stats.ms_mlp += ms_pred;
+      if (use_mlp) ctx->stats.ms_mlp += ms_level;  // the network runs inside the level kernels
       ctx->stats.ms_bh_level += ms_level;
     }
     // every item is exactly one of: leaf pair, far field, split (two children)
@@ -1376,10 +1462,9 @@ bool traverse_device(flk_ctx *ctx, Forest &F, const std::vector<int32_t> &ra, co
   B.levels[0]->reserve(sizeof(BhItem) * cap);
   B.levels[1]->reserve(sizeof(BhItem) * cap);
   B.leaves.reserve(sizeof(BhItem) * cap);
-  if (use_mlp) {
-    mlp_upload(ctx);
-    B.pred.reserve(sizeof(float) * cap);
-  }
+  if (use_mlp) mlp_upload(ctx);
+  const float *mlp_params = use_mlp ? ctx->mlp.d_params.as<float>() : nullptr;
+  const float *mlp_norm = use_mlp ? ctx->mlp.d_norm.as<float>() : nullptr;
   B.roots_a.reserve(sizeof(int32_t) * np);
   B.roots_b.reserve(sizeof(int32_t) * np);
   B.beta2.reserve(sizeof(float) * np);
@@ -1398,7 +1483,7 @@ bool traverse_device(flk_ctx *ctx, Forest &F, const std::vector<int32_t> &ra, co
   hipLaunchKernelGGL(bh_roots, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, B.roots_a.as<int32_t>(),
                      B.roots_b.as<int32_t>(), np, buf[0]);
   FLK_HIP(hipGetLastError());
-  auto lev = ipt == 2 ? bh_level<2> : ipt == 8 ? bh_level<8> : bh_level<4>;
+  const LevelKernel lev = level_kernel(ipt, use_mlp);
   auto leafk = ipt == 2 ? bh_leaf_pairs<2> : ipt == 8 ? bh_leaf_pairs<8> : bh_leaf_pairs<4>;
   const int64_t items_per_block = (int64_t)ipt * kTravBlock;
   const unsigned grid_lev = (unsigned)std::min<int64_t>(resident_grid(ctx, (const void *)lev),
@@ -1408,13 +1493,10 @@ bool traverse_device(flk_ctx *ctx, Forest &F, const std::vector<int32_t> &ra, co
   for (int d = 0; d < D; ++d) {
     const BhItem *in = buf[d & 1];
     const unsigned long long *n_dev = cnt + 4 + 2 * d;
-    if (use_mlp)
-      mlp_predict_items(ctx, in, cap, n_dev, F.geo.as<BhGeo>(), eps, B.pred.as<float>(), cnt + 3, nullptr,
-                        nullptr);
     hipLaunchKernelGGL(lev, dim3(grid_lev), dim3(kTravBlock), 0, s, in, cap, n_dev, F.geo.as<BhGeo>(),
-                       F.mom.as<BhMom>(), (const float *)B.beta2.as<float>(), eps,
-                       (const float *)(use_mlp ? B.pred.as<float>() : nullptr), log_tol, buf[(d + 1) & 1],
-                       cnt + 4 + 2 * (d + 1), cap, B.leaves.as<BhItem>(), cnt + 5, cap, d_sum, d_err);
+                       F.mom.as<BhMom>(), (const float *)B.beta2.as<float>(), eps, mlp_params, mlp_norm, log_tol,
+                       cnt + 3, buf[(d + 1) & 1], cnt + 4 + 2 * (d + 1), cap, B.leaves.as<BhItem>(), cnt + 5, cap,
+                       d_sum, d_err);
     FLK_HIP(hipGetLastError());
   }
   const Curves &cv = ctx->curves;

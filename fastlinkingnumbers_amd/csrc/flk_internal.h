@@ -314,15 +314,6 @@ void mlp_forward(flk_ctx *ctx, const float *x, int64_t n, float *y);
 void mlp_upload(flk_ctx *ctx);
 // Forward pass on device buffers (features n x 12, predictions n).
 void mlp_forward_device(flk_ctx *ctx, const float *dx, int64_t n, float *dy);
-// Learned-criterion predictions for Barnes–Hut work items (log10 error, or
-// +inf where the decision does not depend on it), from the node records.
-struct BhItem;
-struct BhGeo;
-// n items, or with n_dev the count *n_dev (read on the device) bounded by n
-// (+inf where no network is needed); *evals counts the network evaluations.
-void mlp_predict_items(flk_ctx *ctx, const BhItem *items, int64_t n, const unsigned long long *n_dev,
-                       const BhGeo *geo, float eps, float *pred, unsigned long long *evals, hipEvent_t start,
-                       hipEvent_t stop);
 float mlp_gradient(flk_ctx *ctx, const float *x, const float *y, int64_t n, float *grad);
 float mlp_train(flk_ctx *ctx, const float *x, const float *y, int64_t n, int epochs, int batch,
                 float lr, uint64_t seed);

@@ -26,13 +26,13 @@
 
 #include "bh_types.h"
 #include "flk_internal.h"
+#include "mlp_infer.h"
 
 namespace flk {
 
 namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int H = 64;       // hidden width
 constexpr int KP = 32;      // padded input width
@@ -164,120 +164,6 @@ __device__ void stage_x(const BlockLds &L, WaveLds &W, const float *__restrict__
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 }
 
-// Inference: both layers as C = W . X^T (neurons x samples), so lane l ends
-// layer 1 holding neurons nb*16 + 4*(l/16) + i of sample l%16 -- exactly the
-// B fragment of layer 2 once W2's reduction index is permuted to the same
-// order (k-slot 8g + j of step s <-> neuron (2s + j/4)*16 + 4g + j%4).
-// The A fragments of both layers and this lane's output weights live in
-// VGPRs for the whole kernel, the biases and input normalisation in LDS;
-// activations never leave registers.  One tile = 16 samples = 12 MFMAs per
-// wave.
-//
-// FP16 form: inputs, weights and activations are FP16 (v_mfma_f32_16x16x32_f16,
-// FP32 accumulation, the biases entering as the accumulator's initial
-// value).  A pair of layer outputs is rounded to FP16 (v_cvt_pk_f16_f32, to
-// nearest even), its gate min(max(z/6 + 1/2, 0), 1) is one v_pk_fma_f16 with
-// the clamp bit, the activation one v_pk_mul_f16, and the output layer one
-// v_dot2c_f32_f16 per pair (FP32 sum): 2 instructions per activation of
-// layer 1 and 2.5 of layer 2.
-typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
-typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
-typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
-typedef float f32x2 __attribute__((ext_vector_type(2)));
-constexpr float kLog2e = 1.44269504088896340736f;
-constexpr float kLn2 = 0.69314718055994530942f;
-struct InferRegs {
-  f16x8 a1[4], a2[4][2];
-  f16x2 w3[4][2];  // output weights of neurons nb*16 + 4g + {0, 1} and {2, 3}
-};
-struct InferLds {
-  float b1[H], b2[H];
-  float mean[16], istd[16];    // features 12..15 pad: mean 0, istd 0
-  float mean2[16], istd2[16];  // the same for log2 features
-  float b3;
-};
-
-// Hard-swish of two layer outputs in FP16 (one rounding per operation).
-__device__ __forceinline__ f16x2 hswish2(float a, float b) {
-  const f16x2 z = __builtin_convertvector(f32x2{a, b}, f16x2);
-  const f16x2 sixth = {(_Float16)(1.0f / 6.0f), (_Float16)(1.0f / 6.0f)};
-  const f16x2 half = {(_Float16)0.5f, (_Float16)0.5f};
-  const f16x2 gate = __builtin_elementwise_min(
-      __builtin_elementwise_max(__builtin_elementwise_fma(z, sixth, half), f16x2{0, 0}), f16x2{1, 1});
-  return z * gate;
-}
-
-__device__ void load_infer(const float *__restrict__ P, const float *__restrict__ norm, InferRegs &R,
-                           InferLds &L) {
-  const int l = threadIdx.x & 63, g = l >> 4, c = l & 15;
-#pragma unroll
-  for (int nb = 0; nb < 4; ++nb) {
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int f = 8 * g + j;
-      R.a1[nb][j] = (_Float16)(f < FLK_MLP_NFEAT ? P[OFF_W1 + (nb * 16 + c) * FLK_MLP_NFEAT + f] : 0.0f);
-    }
-#pragma unroll
-    for (int st = 0; st < 2; ++st)
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int neuron = (2 * st + j / 4) * 16 + 4 * g + (j & 3);
-        R.a2[nb][st][j] = (_Float16)P[OFF_W2 + (nb * 16 + c) * H + neuron];
-      }
-#pragma unroll
-    for (int q = 0; q < 2; ++q)
-      R.w3[nb][q] = f16x2{(_Float16)P[OFF_W3 + nb * 16 + 4 * g + 2 * q],
-                          (_Float16)P[OFF_W3 + nb * 16 + 4 * g + 2 * q + 1]};
-  }
-  for (int t = threadIdx.x; t < H; t += blockDim.x) {
-    L.b1[t] = P[OFF_B1 + t];
-    L.b2[t] = P[OFF_B2 + t];
-  }
-  for (int t = threadIdx.x; t < 16; t += blockDim.x) {
-    L.mean[t] = t < FLK_MLP_NFEAT ? norm[t] : 0.f;
-    L.istd[t] = t < FLK_MLP_NFEAT ? 1.0f / norm[FLK_MLP_NFEAT + t] : 0.f;
-    // features given as log2 values (bh_mlp_predict): (log2 x - mean / ln 2) * (istd ln 2)
-    L.mean2[t] = t < FLK_MLP_NFEAT ? norm[t] * kLog2e : 0.f;
-    L.istd2[t] = t < FLK_MLP_NFEAT ? kLn2 / norm[FLK_MLP_NFEAT + t] : 0.f;
-  }
-  if (threadIdx.x == 0) L.b3 = P[OFF_B3];
-  __syncthreads();
-}
-
-// One tile: bx is this lane's B fragment of the normalised inputs (features
-// 8g..8g+7 of sample l%16, FP16).  Returns the prediction of sample l%16 in
-// every lane.
-__device__ __forceinline__ float infer_tile(const InferRegs &R, const InferLds &L, f16x8 bx) {
-  const int l = threadIdx.x & 63, g = l >> 4;
-  f16x2 h[4][2];
-#pragma unroll
-  for (int nb = 0; nb < 4; ++nb) {
-    const float4 bb = *reinterpret_cast<const float4 *>(&L.b1[nb * 16 + 4 * g]);
-    const f32x4 z = __builtin_amdgcn_mfma_f32_16x16x32_f16(R.a1[nb], bx, f32x4{bb.x, bb.y, bb.z, bb.w}, 0, 0, 0);
-    h[nb][0] = hswish2(z[0], z[1]);
-    h[nb][1] = hswish2(z[2], z[3]);
-  }
-  // layer 2's B fragments: steps 0 and 1 hold the outputs of blocks 0, 1 and 2, 3
-  const f16x8 bh0 = __builtin_shufflevector(__builtin_shufflevector(h[0][0], h[0][1], 0, 1, 2, 3),
-                                            __builtin_shufflevector(h[1][0], h[1][1], 0, 1, 2, 3), 0, 1, 2,
-                                            3, 4, 5, 6, 7);
-  const f16x8 bh1 = __builtin_shufflevector(__builtin_shufflevector(h[2][0], h[2][1], 0, 1, 2, 3),
-                                            __builtin_shufflevector(h[3][0], h[3][1], 0, 1, 2, 3), 0, 1, 2,
-                                            3, 4, 5, 6, 7);
-  float part = 0.f;
-#pragma unroll
-  for (int nb = 0; nb < 4; ++nb) {
-    const float4 bb = *reinterpret_cast<const float4 *>(&L.b2[nb * 16 + 4 * g]);
-    f32x4 z = __builtin_amdgcn_mfma_f32_16x16x32_f16(R.a2[nb][0], bh0, f32x4{bb.x, bb.y, bb.z, bb.w}, 0, 0, 0);
-    z = __builtin_amdgcn_mfma_f32_16x16x32_f16(R.a2[nb][1], bh1, z, 0, 0, 0);
-    part = __builtin_amdgcn_fdot2(hswish2(z[0], z[1]), R.w3[nb][0], part, false);
-    part = __builtin_amdgcn_fdot2(hswish2(z[2], z[3]), R.w3[nb][1], part, false);
-  }
-  part += __shfl_xor(part, 16, 64);
-  part += __shfl_xor(part, 32, 64);
-  return part + L.b3;
-}
-
 __global__ __launch_bounds__(256) void mlp_forward_kernel(const float *__restrict__ P,
                                                           const float *__restrict__ norm,
                                                           const float *__restrict__ x, int64_t n,
@@ -298,98 +184,9 @@ __global__ __launch_bounds__(256) void mlp_forward_kernel(const float *__restric
           (row < n && f < FLK_MLP_NFEAT) ? (x[row * FLK_MLP_NFEAT + f] - L.mean[f]) * L.istd[f] : 0.f;
       bx[j] = (_Float16)v;
     }
-    const float yv = infer_tile(R, L, bx);
+    const float yv = infer_tile(RegWeights{R}, L, bx);
     if (g == 0 && row < n) y[row] = yv;
   }
-}
-
-// Learned criterion inside the traversal: predictions for the items of a
-// level chunk, straight from the node records.  Each wave takes 64 items per
-// iteration (one per lane): items whose decision cannot depend on the
-// prediction (two leaf buckets, or overlapping bounding spheres, which always
-// split) get +inf without the network; the others' normalised features are
-// compacted into the wave's LDS rows and evaluated 16 per tile.  The next
-// iteration's items and node records are loaded before the tiles run.
-__global__ __launch_bounds__(256) void bh_mlp_predict(const BhItem *__restrict__ items, int64_t n,
-                                                      const unsigned long long *__restrict__ n_dev,
-                                                      const BhGeo *__restrict__ geo, float eps,
-                                                      const float *__restrict__ P,
-                                                      const float *__restrict__ norm,
-                                                      float *__restrict__ pred,
-                                                      unsigned long long *__restrict__ evals) {
-  __shared__ InferLds L;
-  __shared__ __attribute__((aligned(16))) _Float16 xs[4][64][16];
-  __shared__ int32_t src[4][64];
-  __shared__ unsigned int block_evals;  // network evaluations of the block
-  if (n_dev) n = level_count(n_dev, n);     // level enqueued ahead: n is the bound
-  if (n <= 0) return;                      // the whole grid, before any barrier
-  if (threadIdx.x == 0) block_evals = 0;
-  InferRegs R;
-  load_infer(P, norm, R, L);  // ends with a block barrier
-  unsigned int wave_evals = 0;
-  const int l = threadIdx.x & 63, w = threadIdx.x >> 6, g = l >> 4, c = l & 15;
-  const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + w;
-  const int64_t stride = (int64_t)gridDim.x * (blockDim.x >> 6) * 64;
-  int64_t t = wave * 64 + l;
-  BhItem it = t < n ? items[t] : BhItem{0, 0, 0, 0};
-  BhGeo A = geo[it.a], B = geo[it.b];
-  // items run one iteration ahead of the records, so the record loads issued
-  // before the tiles never wait for the item load that addresses them
-  if (t + stride < n) it = items[t + stride];
-  for (int64_t base = wave * 64; base < n; base += stride, t += stride) {
-    bool need = false;
-    if (t < n) {
-      const float rx = (B.cx - A.cx) + (B.lx - A.lx), ry = (B.cy - A.cy) + (B.ly - A.ly),
-                  rz = (B.cz - A.cz) + (B.lz - A.lz);
-      const float dsq = fmaf(rx, rx, fmaf(ry, ry, rz * rz)) + eps;
-      const float rs = A.radius + B.radius;
-      need = !(A.c0 < 0 && B.c0 < 0) && dsq > rs * rs + eps;
-      if (need) {
-        float f[FLK_MLP_NFEAT];
-        mlp_features_t<true>(A, B, dsq, f);  // log2 features, normalised with mean2 / istd2
-        const unsigned long long m = __ballot(true);  // lanes with need == true
-        const int pos = __popcll(m & ((1ull << l) - 1ull));
-        f16x8 v0, v1;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) v0[j] = (_Float16)((f[j] - L.mean2[j]) * L.istd2[j]);
-#pragma unroll
-        for (int j = 0; j < 8; ++j)
-          v1[j] = (_Float16)(j < FLK_MLP_NFEAT - 8 ? (f[8 + j] - L.mean2[8 + j]) * L.istd2[8 + j] : 0.f);
-        *reinterpret_cast<f16x8 *>(&xs[w][pos][0]) = v0;
-        *reinterpret_cast<f16x8 *>(&xs[w][pos][8]) = v1;
-        src[w][pos] = l;
-      } else {
-        pred[t] = __builtin_inff();
-      }
-    }
-    const int cnt = __popcll(__ballot(need));
-    wave_evals += (unsigned int)cnt;
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    // next iteration's records and the item after it, in flight while the
-    // tiles run
-    const int64_t tn = t + stride;
-    if (tn < n) {
-      A = geo[it.a];
-      B = geo[it.b];
-      if (tn + stride < n) it = items[tn + stride];
-    }
-    for (int r0 = 0; r0 < cnt; r0 += 16) {
-      const int row = r0 + c;
-      f16x8 bx;
-      if (g < 2 && row < cnt) bx = *reinterpret_cast<const f16x8 *>(&xs[w][row][8 * g]);
-      else
-#pragma unroll
-        for (int j = 0; j < 8; ++j) bx[j] = (_Float16)0.f;
-      const float yv = infer_tile(R, L, bx);
-      if (g == 0 && row < cnt) pred[base + src[w][row]] = yv;
-    }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  }
-  if (l == 0 && wave_evals) atomicAdd(&block_evals, wave_evals);
-  __syncthreads();
-  if (threadIdx.x == 0 && block_evals) atomicAdd(evals, (unsigned long long)block_evals);
 }
 
 // Forward + backward of the mean squared error over rows idx[0..n) (or 0..n):
@@ -593,16 +390,6 @@ void mlp_forward_device(flk_ctx *ctx, const float *dx, int64_t n, float *dy) {
   const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((n + 63) / 64, 8 * std::max(1, ctx->num_cus)));
   hipLaunchKernelGGL(mlp_forward_kernel, dim3(blocks), dim3(256), 0, ctx->stream,
                      ctx->mlp.d_params.as<float>(), ctx->mlp.d_norm.as<float>(), dx, n, dy);
-  FLK_HIP(hipGetLastError());
-}
-
-void mlp_predict_items(flk_ctx *ctx, const BhItem *items, int64_t n, const unsigned long long *n_dev,
-                       const BhGeo *geo, float eps, float *pred, unsigned long long *evals, hipEvent_t start,
-                       hipEvent_t stop) {
-  if (n <= 0) return;
-  const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8 * std::max(1, ctx->num_cus)));
-  launch_timed(start, stop, bh_mlp_predict, dim3(blocks), dim3(256), ctx->stream, items, n, n_dev, geo, eps,
-               ctx->mlp.d_params.as<float>(), ctx->mlp.d_norm.as<float>(), pred, evals);
   FLK_HIP(hipGetLastError());
 }
 

@@ -60,8 +60,8 @@ typedef struct flk_params {
   int32_t bh_batch_ksegs; /* Barnes–Hut pairs are processed in batches whose
                              curves total at most this many thousand segments
                              (0: sized from free device memory) */
-  int32_t timing;       /* 1: time every Barnes–Hut level and prediction launch
-                           with HIP events (ms_bh_level, ms_mlp); the per-launch
+  int32_t timing;       /* 1: time every Barnes–Hut level launch with HIP events
+                           (ms_bh_level, ms_mlp); the per-launch
                            events cost ~15% of a traversal, so 0 by default */
   int32_t graph;        /* 1 (default): a direct-sum batch is captured in a HIP
                            graph and replayed when the same pairs come again;
@@ -86,7 +86,8 @@ typedef struct flk_stats {
   int32_t ds_launches;      /* direct-sum tile kernel launches */
   int32_t ds_waves;         /* waves per direct-sum launch */
   double ms_bh_level;       /* device time of the traversal level kernels (params.timing) */
-  double ms_mlp;            /* device time of the learned-criterion predictions (params.timing) */
+  double ms_mlp;            /* device time of the level kernels that evaluated the learned criterion,
+                               which runs inside them (params.timing) */
   int64_t mlp_evals;        /* learned-criterion network evaluations */
   int64_t bh_host_passes;   /* Barnes–Hut passes that ran level by level from the host
                                (per-launch timing, or the device-scheduled pass's
