@@ -1244,6 +1244,16 @@ void build_forest(flk_ctx *ctx, const std::vector<int32_t> &curves_in, int leaf,
   for (int32_t k = 0; k < nc; ++k) F.root[cs[k]] = k;  // roots are nodes 0..nc-1
 }
 
+// Items per thread of the level and leaf kernels: the tuned value, or by
+// criterion (0): 8 for beta, 4 for the learned criterion, whose network
+// staging in LDS would hold 8 items per thread to 2 blocks per CU (measured
+// on 2 x 8M random walks: beta 21.4 ms at 8 against 21.8 at 4, learned 12.5
+// at 4 against 15.1 at 8).
+int items_per_thread(const flk_ctx *ctx, bool use_mlp) {
+  const int t = ctx->tune.bh_items_per_thread;
+  return (t == 2 || t == 4 || t == 8) ? t : (use_mlp ? 4 : 8);
+}
+
 // The level kernel for a criterion and a number of items per thread.
 using LevelKernel = void (*)(const BhItem *, int64_t, const unsigned long long *, const BhGeo *, const BhMom *,
                              const float *, float, const float *, const float *, float, unsigned long long *,
@@ -1275,9 +1285,7 @@ void traverse(flk_ctx *ctx, Forest &F, const std::vector<int32_t> &ra, const std
   constexpr int kSlots = 1024;  // count pairs per zeroing of the counter array
   constexpr int kRound = 4;     // levels per host round trip (at most)
   // items per thread of the level/leaf kernels: 2, 4 or 8 (grid and kernel agree)
-  const int ipt = (ctx->tune.bh_items_per_thread == 2 || ctx->tune.bh_items_per_thread == 8)
-                      ? ctx->tune.bh_items_per_thread
-                      : 4;
+  const int ipt = items_per_thread(ctx, use_mlp);
   B.roots_a.reserve(sizeof(int32_t) * np);
   B.roots_b.reserve(sizeof(int32_t) * np);
   B.beta2.reserve(sizeof(float) * np);
@@ -1455,9 +1463,7 @@ bool traverse_device(flk_ctx *ctx, Forest &F, const std::vector<int32_t> &ra, co
   }
   const int64_t cap = ctx->tune.bh_level_capacity > 0 ? (int64_t)ctx->tune.bh_level_capacity : B.dev_cap;
   if (np > cap) return false;
-  const int ipt = (ctx->tune.bh_items_per_thread == 2 || ctx->tune.bh_items_per_thread == 8)
-                      ? ctx->tune.bh_items_per_thread
-                      : 4;
+  const int ipt = items_per_thread(ctx, use_mlp);
   while (B.levels.size() < 2) B.levels.push_back(std::make_unique<DevBuf>());
   B.levels[0]->reserve(sizeof(BhItem) * cap);
   B.levels[1]->reserve(sizeof(BhItem) * cap);

@@ -515,8 +515,9 @@ int32_t flk_set_tune(flk_ctx *ctx, const flk_tune *tune) {
       throw Error(FLK_ERR_ARG, "ds_blocks_per_cu must be in [0, 64]");
     if (tune->ds_tasks_per_wave < 1 || tune->ds_tasks_per_wave > 1024)
       throw Error(FLK_ERR_ARG, "ds_tasks_per_wave must be in [1, 1024]");
-    if (tune->bh_items_per_thread != 2 && tune->bh_items_per_thread != 4 && tune->bh_items_per_thread != 8)
-      throw Error(FLK_ERR_ARG, "bh_items_per_thread must be 2, 4 or 8");
+    if (tune->bh_items_per_thread != 0 && tune->bh_items_per_thread != 2 && tune->bh_items_per_thread != 4 &&
+        tune->bh_items_per_thread != 8)
+      throw Error(FLK_ERR_ARG, "bh_items_per_thread must be 0 (by criterion), 2, 4 or 8");
     if (tune->ingest_threads < 0 || tune->ingest_threads > 64)
       throw Error(FLK_ERR_ARG, "ingest_threads must be in [0, 64]");
     if (tune->bh_level_capacity != 0 && tune->bh_level_capacity < 64)

@@ -231,7 +231,7 @@ struct flk_ctx {
   flk::HostStage stage;
   // default shapes: measured best on the 1,600-loop chain mail (R = 2 with
   // every block resident: 880 ms against 931 ms for R = 4, 1,068 ms for R = 8)
-  flk_tune tune{2, 0, 8, 4, 0, 0, {0, 0}};
+  flk_tune tune{2, 0, 8, 0, 0, 0, {0, 0}};
   flk::DsGraph ds_graph[2];      // FP32, FP64 batches
   flk::DiscResult disc;          // last flk_discretize result
   flk::BccLoad bcc;              // last flk_load_bcc result

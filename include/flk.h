@@ -274,7 +274,8 @@ typedef struct flk_tune {
   int32_t ds_blocks_per_cu;    /* persistent direct-sum grid: 256-thread blocks
                                   per CU (0: as many as are resident at once) */
   int32_t ds_tasks_per_wave;   /* column chunking: target tasks per wave */
-  int32_t bh_items_per_thread; /* Barnes–Hut level and leaf kernels: 2, 4 or 8 */
+  int32_t bh_items_per_thread; /* Barnes–Hut level and leaf kernels: 2, 4 or 8 (0: by
+                                  criterion, 8 for beta and 4 for the learned one) */
   int32_t ingest_threads;      /* host threads filling the upload's pinned
                                   staging buffers (0: half the cores, at most 8) */
   int32_t bh_level_capacity;   /* Barnes–Hut items per level buffer when all levels
