@@ -208,7 +208,8 @@ def main():
         kern.append(res.stats["ms_ds_kernel"])
         log(f"step {len(times)}: {times[-1] * 1e3:.1f} ms (engine {res.stats['ms_total']:.1f} ms: direct-sum "
             f"kernel {kern[-1]:.1f} ms, bh build {res.stats['ms_bh_build']:.1f} ms, traverse "
-            f"{res.stats['ms_bh_traverse']:.1f} ms, visits {res.stats['bh_visits']}, fallback "
+            f"{res.stats['ms_bh_traverse']:.1f} ms, graph launches {res.stats['graph_launches']} "
+            f"({res.stats['ms_kernel']:.1f} ms device), visits {res.stats['bh_visits']}, fallback "
             f"{res.stats['ms_fallback']:.1f} ms; pls {phase['pls'] * 1e3:.1f} ms, rounding "
             f"{phase['round'] * 1e3:.1f} ms)")
     st = res.stats
@@ -401,17 +402,21 @@ def learned_vs_directsum(eng, flk, workloads, args):
     P, nm = criterion.load_weights()
     eng.mlp_set_weights(P, nm)
     for name, kw in (("beta", {}), ("learned", {"criterion": "mlp"})):
-        eng.linking_numbers([[0, 1]], method="barneshut", **kw)  # warm-up
-        r = eng.linking_numbers([[0, 1]], method="barneshut", **kw)
+        eng.linking_numbers([[0, 1]], method="barneshut", **kw)  # warm-up (captures the graph)
+        r = eng.linking_numbers([[0, 1]], method="barneshut", **kw)  # one graph launch
         st = r.stats
+        # the same certificate launched eagerly, for the build / traversal split
+        eager = eng.linking_numbers([[0, 1]], method="barneshut", graph=0, **kw).stats
         dev = eng.linking_numbers([[0, 1]], method="barneshut", timing=1, **kw).stats  # kernel times
         res[name] = {"ms": round(st["ms_total"], 1), "lk": float(r.lk[0]),
+                     "graph_launches": int(st["graph_launches"]), "ms_graph_device": round(st["ms_kernel"], 2),
+                     "ms_eager": round(eager["ms_total"], 1),
                      "off_integer": float(abs(r.lk[0] - round(r.lk[0]))),
                      "err_estimate": float(r.err[0]),
                      "visits": int(st["bh_visits"]), "far": int(st["bh_far"]),
                      "leaf_pairs": int(st["bh_leaf_pairs"]), "mlp_evals": int(st.get("mlp_evals", 0)),
                      "retries": int(st["retries"]), "fallbacks": int(st["fallbacks"]),
-                     "ms_build": round(st["ms_bh_build"], 2), "ms_traverse": round(st["ms_bh_traverse"], 2),
+                     "ms_build": round(eager["ms_bh_build"], 2), "ms_traverse": round(eager["ms_bh_traverse"], 2),
                      "ms_level_kernels": round(dev["ms_bh_level"], 2), "ms_mlp_kernels": round(dev["ms_mlp"], 2)}
         if "ds_lk" in res:
             res[name]["abs_diff_vs_ds"] = float(abs(r.lk[0] - res["ds_lk"]))

@@ -75,7 +75,9 @@ class FlkStats(C.Structure):
         ("ms_mlp", C.c_double),
         ("mlp_evals", C.c_int64),
         ("bh_host_passes", C.c_int64),
-        ("reserved", C.c_int64 * 4),
+        ("graph_launches", C.c_int64),
+        ("ms_graph_launch", C.c_double),
+        ("reserved", C.c_int64 * 2),
     ]
 
     def as_dict(self) -> dict:

@@ -1042,6 +1042,109 @@ __global__ void bh_pass_init(unsigned long long *__restrict__ cnt, int n, int ro
   if (k < n) cnt[k] = k == root_slot ? roots : 0ull;
 }
 
+__global__ void bh_zero_f64(double *__restrict__ p, int64_t n) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = 0.0;
+}
+
+// Exclusive scan of one value per thread over a 256-thread block; *total
+// receives the sum.  sw: 4 ints of LDS.  Contains block barriers.
+__device__ __forceinline__ int block_scan256(int v, int *total, int *sw) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int inc = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int u = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += u;
+  }
+  if (lane == 63) sw[w] = inc;
+  __syncthreads();
+  int off = 0, tot = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    off += k < w ? sw[k] : 0;
+    tot += sw[k];
+  }
+  __syncthreads();
+  *total = tot;
+  return off + inc - v;
+}
+
+// The re-run decision after a batch's first pass, on the device.  Beta
+// (reference linking_number_certificate.cpp:165-184): the estimate scaled by
+// err_scale gives the target beta = min(beta_init (E / target)^(1/4),
+// beta_limit), and a pair whose target exceeds beta_init is re-run with it.
+// Learned criterion: the root-sum-square of the accepted predictions is the
+// estimate, and a pair above a tenth of the target is re-run (one decade
+// stricter, by the caller's second pass).  One block walks the pairs in order
+// and compacts the re-runs with a block scan, so the list is deterministic:
+// root items (pair = position in the list), beta^2 and the pair's index.  It
+// also clears the second pass's counters and sets its root count at
+// cnt2[root_slot].
+__global__ __launch_bounds__(256) void bh_adapt(int64_t nb, const int32_t *__restrict__ ra,
+                                                const int32_t *__restrict__ rb, const double *__restrict__ err1,
+                                                int mlp, double err_scale, double target_error, double beta_init,
+                                                double beta_limit, BhItem *__restrict__ items2,
+                                                float *__restrict__ beta2_2, int32_t *__restrict__ redo,
+                                                unsigned long long *__restrict__ cnt2, int nslots2, int root_slot) {
+  __shared__ int sw[4];
+  for (int k = threadIdx.x; k < nslots2; k += 256) cnt2[k] = 0ull;
+  int carry = 0;
+  for (int64_t c0 = 0; c0 < nb; c0 += 256) {
+    const int64_t t = c0 + threadIdx.x;
+    int flag = 0;
+    float b2 = 0.f;
+    if (t < nb) {
+      if (!mlp) {
+        const double want = pow(err1[t] * err_scale / target_error, 0.25) * beta_init;
+        const double target = beta_limit < want ? beta_limit : want;  // std::min: a NaN estimate stays NaN
+        if (target > beta_init) {
+          flag = 1;
+          b2 = (float)(target * target);
+        }
+      } else if (sqrt(err1[t]) > 0.1 * target_error) {
+        flag = 1;
+        b2 = (float)(beta_init * beta_init);
+      }
+    }
+    int tot = 0;
+    const int pos = carry + block_scan256(flag, &tot, sw);
+    if (flag) {
+      items2[pos] = BhItem{pos, ra[t], rb[t], 0};
+      beta2_2[pos] = b2;
+      redo[pos] = (int32_t)t;
+    }
+    carry += tot;
+  }
+  __syncthreads();  // the clear above is done before the count is set
+  if (threadIdx.x == 0) cnt2[root_slot] = (unsigned long long)carry;
+}
+
+// Per-pair results of a batch: the first pass's sum and estimate (scaled by
+// err_scale for beta, the root-sum-square for the learned criterion) ...
+__global__ void bh_merge_first(int64_t nb, const double *__restrict__ sum, const double *__restrict__ err, int mlp,
+                               double err_scale, double *__restrict__ v, double *__restrict__ e,
+                               uint32_t *__restrict__ f) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nb) return;
+  v[t] = sum[t];
+  e[t] = mlp ? sqrt(err[t]) : err[t] * err_scale;
+  f[t] = 0u;
+}
+
+// ... replaced by the second pass's for the re-run pairs.
+__global__ void bh_merge_redo(int64_t nb, const unsigned long long *__restrict__ n_redo,
+                              const int32_t *__restrict__ redo, const double *__restrict__ sum2,
+                              const double *__restrict__ err2, int mlp, double err_scale, double *__restrict__ v,
+                              double *__restrict__ e, uint32_t *__restrict__ f) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nb || k >= (int64_t)*n_redo) return;
+  const int32_t t = redo[k];
+  v[t] = sum2[k];
+  e[t] = mlp ? sqrt(err2[k]) : err2[k] * err_scale;
+  f[t] = FLK_PAIR_RETRY;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- host side
@@ -1060,21 +1163,72 @@ struct Forest {
   explicit Forest(BhScratch &b) : geo(b.geo), mom(b.mom), order(b.order) {}
 };
 
-void build_forest(flk_ctx *ctx, const std::vector<int32_t> &curves_in, int leaf, Forest &F) {
-  Curves &cv = ctx->curves;
-  hipStream_t s = ctx->stream;
-  F.root.assign(cv.ncurves, -1);
+// Host plan of a forest build: the curves with trees (rank order), their
+// segment and vertex offsets, key boxes, roots and the level structure, all
+// known before anything runs on the device.  plan_forest also reserves every
+// buffer the build writes; enqueue_forest then only enqueues (no allocation,
+// no read-back), so the build can be captured.  The host arrays are read by
+// the build's copies when they run: the plan outlives them.
+struct ForestPlan {
+  int leaf = 1, direct = kDirectSegments;
   std::vector<int32_t> cs;
+  std::vector<int64_t> seg_off, vtx_off;
+  std::vector<double> cbox;
+  std::vector<BhRange> roots;
+  int64_t S = 0;
+  int end_bit = 0;
+  bool key32 = true;
+  std::vector<int32_t> level_base, level_n;
+  std::vector<int64_t> level_split, level_max;
+  int32_t total = 0;
+  int lev_top = 0;
+  // In a graph: seg_off, vtx_off, cbox and roots uploaded once (dsrc, in that
+  // order), copied device to device on every replay.
+  DevBuf dconst;
+  std::vector<const void *> dsrc;
+};
+
+// Uploads host arrays into one device buffer, each piece 16-byte aligned, and
+// returns the pieces' device addresses (the graph's copies then read these).
+std::vector<const void *> upload_pack(DevBuf &dst, const std::vector<std::pair<const void *, size_t>> &parts,
+                                      hipStream_t s) {
+  size_t total = 0;
+  for (const auto &p : parts) total += (p.second + 15) & ~size_t(15);
+  dst.reserve(std::max<size_t>(total, 16));
+  std::vector<const void *> out;
+  size_t off = 0;
+  for (const auto &p : parts) {
+    unsigned char *d = dst.as<unsigned char>() + off;
+    if (p.second) FLK_HIP(hipMemcpyAsync(d, p.first, p.second, hipMemcpyHostToDevice, s));
+    out.push_back(d);
+    off += (p.second + 15) & ~size_t(15);
+  }
+  return out;
+}
+
+// A plan's host array into device scratch: device to device from its
+// uploaded copy (a graph), else from the host.
+void put(void *dst, const void *host, const std::vector<const void *> &dsrc, int i, size_t bytes, hipStream_t s) {
+  if (dsrc.empty()) FLK_HIP(hipMemcpyAsync(dst, host, bytes, hipMemcpyHostToDevice, s));
+  else FLK_HIP(hipMemcpyAsync(dst, dsrc[i], bytes, hipMemcpyDeviceToDevice, s));
+}
+
+void plan_forest(flk_ctx *ctx, const std::vector<int32_t> &curves_in, int leaf, ForestPlan &P) {
+  Curves &cv = ctx->curves;
+  P = ForestPlan{};
+  P.leaf = leaf;
+  P.direct = std::max(leaf, kDirectSegments);
   for (int32_t c : curves_in)
-    if (cv.n[c] >= 2) cs.push_back(c);
-  const int32_t nc = (int32_t)cs.size();
+    if (cv.n[c] >= 2) P.cs.push_back(c);
+  const int32_t nc = (int32_t)P.cs.size();
   if (nc == 0) return;
   if (cv.nvert_dev >= (int64_t)UINT32_MAX) throw Error(FLK_ERR_ARG, "too many vertices for Barnes-Hut");
-  std::vector<int64_t> seg_off(nc + 1, 0), vtx_off(nc);
-  std::vector<double> cbox(6 * (size_t)nc);
+  P.seg_off.assign(nc + 1, 0);
+  P.vtx_off.assign(nc, 0);
+  P.cbox.assign(6 * (size_t)nc, 0.0);
   for (int32_t k = 0; k < nc; ++k) {
-    seg_off[k + 1] = seg_off[k] + cv.n[cs[k]];
-    vtx_off[k] = cv.off[cs[k]];
+    P.seg_off[k + 1] = P.seg_off[k] + cv.n[P.cs[k]];
+    P.vtx_off[k] = cv.off[P.cs[k]];
   }
   // Sort keys: curve rank and a Morton code of 10 bits per axis relative to
   // the curve's box (a curve's tree never depends on the batch around it);
@@ -1082,78 +1236,32 @@ void build_forest(flk_ctx *ctx, const std::vector<int32_t> &curves_in, int leaf,
   int curve_bits = 0;
   while ((1LL << curve_bits) < nc) ++curve_bits;
   const int axis_bits = 10;
-  const int end_bit = 3 * axis_bits + curve_bits;
-  const bool key32 = end_bit <= 32;
+  P.end_bit = 3 * axis_bits + curve_bits;
+  P.key32 = P.end_bit <= 32;
   const double cells = (double)(1 << axis_bits);
   for (int32_t k = 0; k < nc; ++k)
     for (int d = 0; d < 3; ++d) {
-      const double lo = (cv.box[6 * cs[k] + d] - cv.origin[d]) * cv.scale;
-      const double hi = (cv.box[6 * cs[k] + 3 + d] - cv.origin[d]) * cv.scale;
-      cbox[6 * k + d] = lo;
-      cbox[6 * k + 3 + d] = hi > lo ? cells / (hi - lo) : 0.0;
+      const double lo = (cv.box[6 * P.cs[k] + d] - cv.origin[d]) * cv.scale;
+      const double hi = (cv.box[6 * P.cs[k] + 3 + d] - cv.origin[d]) * cv.scale;
+      P.cbox[6 * k + d] = lo;
+      P.cbox[6 * k + 3 + d] = hi > lo ? cells / (hi - lo) : 0.0;
     }
-  const int64_t S = seg_off[nc];
+  const int64_t S = P.S = P.seg_off[nc];
   if (S >= INT32_MAX / 2) throw Error(FLK_ERR_ARG, "too many Barnes-Hut segments");
-  BhScratch &B = ctx->bh;
-  DevBuf &d_seg = B.seg, &d_vtx = B.vtx, &d_box = B.box, &d_k0 = B.keys0, &d_k1 = B.keys1, &d_v0 = B.ids0,
-         &d_tmp = B.tmp;
-  d_seg.reserve(sizeof(int64_t) * (nc + 1));
-  d_vtx.reserve(sizeof(int64_t) * nc);
-  d_box.reserve(sizeof(double) * 6 * nc);
-  d_k0.reserve(sizeof(uint64_t) * S);
-  d_k1.reserve(sizeof(uint64_t) * S);
-  d_v0.reserve(sizeof(uint32_t) * S);
-  F.order.reserve(sizeof(uint32_t) * S);
-  FLK_HIP(hipMemcpyAsync(d_seg.ptr, seg_off.data(), sizeof(int64_t) * (nc + 1), hipMemcpyHostToDevice, s));
-  FLK_HIP(hipMemcpyAsync(d_vtx.ptr, vtx_off.data(), sizeof(int64_t) * nc, hipMemcpyHostToDevice, s));
-  FLK_HIP(hipMemcpyAsync(d_box.ptr, cbox.data(), sizeof(double) * 6 * nc, hipMemcpyHostToDevice, s));
-  auto sort_keys = [&](auto *k0, auto *k1) {
-    using K = std::remove_pointer_t<decltype(k0)>;
-    hipLaunchKernelGGL(bh_keys<K>, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, s,
-                       cv.dpts.as<double4>(), d_seg.as<int64_t>(), d_vtx.as<int64_t>(), d_box.as<double>(),
-                       nc, axis_bits, S, k0, d_v0.as<uint32_t>());
-    FLK_HIP(hipGetLastError());
-    size_t tb = 0;
-    FLK_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k1, d_v0.as<uint32_t>(), F.order.as<uint32_t>(),
-                                               (int)S, 0, end_bit, s));
-    d_tmp.reserve(std::max<size_t>(tb, 16));
-    FLK_HIP(hipcub::DeviceRadixSort::SortPairs(d_tmp.ptr, tb, k0, k1, d_v0.as<uint32_t>(), F.order.as<uint32_t>(),
-                                               (int)S, 0, end_bit, s));
-  };
-  if (key32) sort_keys(d_k0.as<uint32_t>(), d_k1.as<uint32_t>());
-  else sort_keys(d_k0.as<uint64_t>(), d_k1.as<uint64_t>());
-  DevBuf &d_sp = B.sp;
-  d_sp.reserve(sizeof(double2) * 3 * S);
-  hipLaunchKernelGGL(bh_stage_segments, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, s,
-                     F.order.as<uint32_t>(), cv.dpts.as<double4>(), S, d_sp.as<double2>());
-  FLK_HIP(hipGetLastError());
-
-  // level-by-level node creation
-  std::vector<int32_t> level_base{0}, level_n{nc};
-  std::vector<BhRange> roots(nc);
-  for (int32_t k = 0; k < nc; ++k) roots[k] = BhRange{(int32_t)seg_off[k], (int32_t)seg_off[k + 1]};
-  const int64_t max_nodes = 2 * S + nc;
-  DevBuf &d_range = B.range, &d_child = B.child, &d_flag = B.flag, &d_scan = B.scan;
-  d_range.reserve(sizeof(BhRange) * max_nodes);
-  d_child.reserve(sizeof(int2) * max_nodes);
-  d_flag.reserve(sizeof(int32_t) * (S + nc + 1));
-  d_scan.reserve(sizeof(int32_t) * (S + nc + 1));
-  FLK_HIP(hipMemcpyAsync(d_range.ptr, roots.data(), sizeof(BhRange) * nc, hipMemcpyHostToDevice, s));
+  P.roots.resize(nc);
+  for (int32_t k = 0; k < nc; ++k) P.roots[k] = BhRange{(int32_t)P.seg_off[k], (int32_t)P.seg_off[k + 1]};
   // Splits per level from the segment counts alone: a range of m > leaf
   // segments splits into floor(m/2) and m - floor(m/2), so every level of a
   // curve's tree holds at most two distinct range sizes (m and m + 1).  With
   // the counts known, the levels are enqueued without reading anything back.
-  const int32_t direct = std::max(leaf, kDirectSegments);
-  std::vector<int64_t> level_split;
-  std::vector<int64_t> level_max;  // largest range of each level
   {
     std::vector<std::pair<int64_t, int64_t>> sizes;  // (range size, count) of the current level
     std::vector<std::pair<int64_t, int64_t>> next;
-    for (int32_t k = 0; k < nc; ++k) sizes.emplace_back(seg_off[k + 1] - seg_off[k], 1);
+    for (int32_t k = 0; k < nc; ++k) sizes.emplace_back(P.seg_off[k + 1] - P.seg_off[k], 1);
     for (;;) {
       int64_t split = 0, mx = 0;
       for (const auto &sc : sizes) mx = std::max(mx, sc.first);
-      level_max.push_back(mx);
+      P.level_max.push_back(mx);
       next.clear();
       for (const auto &sc : sizes) {
         if (sc.first <= leaf) continue;
@@ -1161,7 +1269,7 @@ void build_forest(flk_ctx *ctx, const std::vector<int32_t> &curves_in, int leaf,
         next.emplace_back(sc.first / 2, sc.second);
         next.emplace_back(sc.first - sc.first / 2, sc.second);
       }
-      level_split.push_back(split);
+      P.level_split.push_back(split);
       if (split == 0) break;
       // merge equal sizes (keeps the list at two entries per distinct size pair)
       std::sort(next.begin(), next.end());
@@ -1172,76 +1280,117 @@ void build_forest(flk_ctx *ctx, const std::vector<int32_t> &curves_in, int leaf,
       }
     }
   }
-  // scan scratch for the widest level, reserved once: nothing is read back
-  // between levels, so a buffer must not be replaced under queued scans
-  {
-    int64_t widest = nc;
-    for (int64_t sp : level_split) widest = std::max(widest, 2 * sp);
-    size_t tb = 0;
-    FLK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, d_flag.as<int32_t>(), d_scan.as<int32_t>(),
-                                             (int)widest, s));
-    FLK_HIP(hipStreamSynchronize(s));
-    d_tmp.reserve(std::max<size_t>(tb, 16));
-  }
+  P.level_base.assign(1, 0);
+  P.level_n.assign(1, nc);
   int32_t total = nc;
-  for (size_t lev = 0; lev < level_split.size(); ++lev) {
-    const int32_t base = level_base[lev], n = level_n[lev];
-    const int32_t nsplit = (int32_t)level_split[lev];
-    const unsigned nb = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL(bh_split_flags, dim3(nb), dim3(256), 0, s, d_range.as<BhRange>() + base, n,
-                       leaf, d_flag.as<int32_t>());
-    size_t tbl = 0;
-    FLK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tbl, d_flag.as<int32_t>(), d_scan.as<int32_t>(), n, s));
-    if (tbl > d_tmp.bytes) throw Error(FLK_ERR_STATE, "scan scratch too small");
-    FLK_HIP(hipcub::DeviceScan::ExclusiveSum(d_tmp.ptr, tbl, d_flag.as<int32_t>(), d_scan.as<int32_t>(), n, s));
-    hipLaunchKernelGGL(bh_split, dim3(nb), dim3(256), 0, s, d_range.as<BhRange>() + base, n, base,
-                       total, d_flag.as<int32_t>(), d_scan.as<int32_t>(),
-                       d_range.as<BhRange>() + total, d_child.as<int2>());
-    FLK_HIP(hipGetLastError());
+  for (size_t lev = 0; lev < P.level_split.size(); ++lev) {
+    const int32_t nsplit = (int32_t)P.level_split[lev];
     if (nsplit == 0) break;
-    level_base.push_back(total);
-    level_n.push_back(2 * nsplit);
+    P.level_base.push_back(total);
+    P.level_n.push_back(2 * nsplit);
     total += 2 * nsplit;
   }
-  F.nnodes = total;
-  F.depth = (int32_t)level_base.size();
+  P.total = total;
+  // the bottom levels (every range within `direct`) run as one launch
+  P.lev_top = (int)P.level_base.size();
+  for (int lev = 0; lev < (int)P.level_base.size() && lev < (int)P.level_max.size(); ++lev)
+    if (P.level_max[lev] <= P.direct) {
+      P.lev_top = lev;
+      break;
+    }
   // FP64 records exist only for nodes above `direct` segments: node ids are
   // assigned level by level, so they all fall below the first level whose
   // ranges are all that small
-  int64_t m64_nodes = total;
-  for (size_t lev = 0; lev < level_base.size() && lev < level_max.size(); ++lev)
-    if (level_max[lev] <= direct) {
-      m64_nodes = level_base[lev];
-      break;
-    }
-  DevBuf &d_m64 = B.m64;
-  d_m64.reserve(sizeof(Mom64) * std::max<int64_t>(m64_nodes, 1));
-  // node records of every level, sized before any moment kernel writes them
-  F.geo.reserve(sizeof(BhGeo) * total);
-  F.mom.reserve(sizeof(BhMom) * total);
-  // the bottom levels (every range within `direct`) in one launch
-  int lev_top = (int)level_base.size();
-  for (int lev = 0; lev < (int)level_base.size() && lev < (int)level_max.size(); ++lev)
-    if (level_max[lev] <= direct) {
-      lev_top = lev;
-      break;
-    }
-  if (lev_top < (int)level_base.size()) {
-    const int32_t b0 = level_base[lev_top], nd = total - b0;
+  const int64_t m64_nodes = P.lev_top < (int)P.level_base.size() ? P.level_base[P.lev_top] : total;
+  int64_t widest = nc;
+  for (int64_t sp : P.level_split) widest = std::max(widest, 2 * sp);
+  BhScratch &B = ctx->bh;
+  B.seg.reserve(sizeof(int64_t) * (nc + 1));
+  B.vtx.reserve(sizeof(int64_t) * nc);
+  B.box.reserve(sizeof(double) * 6 * nc);
+  B.keys0.reserve((P.key32 ? 4 : 8) * S);
+  B.keys1.reserve((P.key32 ? 4 : 8) * S);
+  B.ids0.reserve(sizeof(uint32_t) * S);
+  B.order.reserve(sizeof(uint32_t) * S);
+  B.flag.reserve(sizeof(int32_t) * std::max<int64_t>(S + nc + 1, widest));
+  B.scan.reserve(sizeof(int32_t) * std::max<int64_t>(S + nc + 1, widest));  // also the sort's value scratch
+  B.tmp.reserve(std::max(radix_temp_bytes(S), scan_temp_bytes(widest)));
+  B.sp.reserve(sizeof(double2) * 3 * S);
+  B.range.reserve(sizeof(BhRange) * (2 * S + nc));
+  B.child.reserve(sizeof(int2) * (2 * S + nc));
+  B.m64.reserve(sizeof(Mom64) * std::max<int64_t>(m64_nodes, 1));
+  B.geo.reserve(sizeof(BhGeo) * total);
+  B.mom.reserve(sizeof(BhMom) * total);
+}
+
+void enqueue_forest(flk_ctx *ctx, const ForestPlan &P, hipStream_t s) {
+  const int32_t nc = (int32_t)P.cs.size();
+  if (nc == 0) return;
+  Curves &cv = ctx->curves;
+  BhScratch &B = ctx->bh;
+  const int64_t S = P.S;
+  put(B.seg.ptr, P.seg_off.data(), P.dsrc, 0, sizeof(int64_t) * (nc + 1), s);
+  put(B.vtx.ptr, P.vtx_off.data(), P.dsrc, 1, sizeof(int64_t) * nc, s);
+  put(B.box.ptr, P.cbox.data(), P.dsrc, 2, sizeof(double) * 6 * nc, s);
+  auto sort_keys = [&](auto *k0, auto *k1) {
+    using K = std::remove_pointer_t<decltype(k0)>;
+    hipLaunchKernelGGL(bh_keys<K>, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, s, cv.dpts.as<double4>(),
+                       B.seg.as<int64_t>(), B.vtx.as<int64_t>(), B.box.as<double>(), nc, 10, S, k0,
+                       B.ids0.as<uint32_t>());
+    FLK_HIP(hipGetLastError());
+    radix_sort_pairs<K>(k0, k1, B.ids0.as<uint32_t>(), B.scan.as<uint32_t>(), B.order.as<uint32_t>(), S, P.end_bit,
+                        B.tmp.ptr, s);
+  };
+  if (P.key32) sort_keys(B.keys0.as<uint32_t>(), B.keys1.as<uint32_t>());
+  else sort_keys(B.keys0.as<uint64_t>(), B.keys1.as<uint64_t>());
+  hipLaunchKernelGGL(bh_stage_segments, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, s, B.order.as<uint32_t>(),
+                     cv.dpts.as<double4>(), S, B.sp.as<double2>());
+  FLK_HIP(hipGetLastError());
+  // level-by-level node creation
+  put(B.range.ptr, P.roots.data(), P.dsrc, 3, sizeof(BhRange) * nc, s);
+  for (size_t lev = 0; lev < P.level_base.size(); ++lev) {
+    const int32_t base = P.level_base[lev], n = P.level_n[lev];
+    const int32_t next = lev + 1 < P.level_base.size() ? P.level_base[lev + 1] : P.total;
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(bh_split_flags, dim3(nb), dim3(256), 0, s, B.range.as<BhRange>() + base, n, P.leaf,
+                       B.flag.as<int32_t>());
+    FLK_HIP(hipGetLastError());
+    exclusive_scan(B.flag.as<int32_t>(), B.scan.as<int32_t>(), n, B.tmp.as<int32_t>(), s);
+    hipLaunchKernelGGL(bh_split, dim3(nb), dim3(256), 0, s, B.range.as<BhRange>() + base, n, base, next,
+                       B.flag.as<int32_t>(), B.scan.as<int32_t>(), B.range.as<BhRange>() + next,
+                       B.child.as<int2>());
+    FLK_HIP(hipGetLastError());
+  }
+  if (P.lev_top < (int)P.level_base.size()) {
+    const int32_t b0 = P.level_base[P.lev_top], nd = P.total - b0;
     hipLaunchKernelGGL(bh_moments_direct, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, s,
-                       d_range.as<BhRange>(), b0, nd, d_child.as<int2>(), d_sp.as<double2>(),
-                       F.geo.as<BhGeo>(), F.mom.as<BhMom>());
+                       B.range.as<BhRange>(), b0, nd, B.child.as<int2>(), B.sp.as<double2>(), B.geo.as<BhGeo>(),
+                       B.mom.as<BhMom>());
     FLK_HIP(hipGetLastError());
   }
-  for (int lev = lev_top - 1; lev >= 0; --lev) {
-    const int32_t n = level_n[lev];
-    hipLaunchKernelGGL(bh_moments, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, d_range.as<BhRange>(),
-                       n, level_base[lev], direct, d_child.as<int2>(), d_sp.as<double2>(),
-                       d_m64.as<Mom64>(), F.geo.as<BhGeo>(), F.mom.as<BhMom>());
+  for (int lev = P.lev_top - 1; lev >= 0; --lev) {
+    const int32_t n = P.level_n[lev];
+    hipLaunchKernelGGL(bh_moments, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, B.range.as<BhRange>(), n,
+                       P.level_base[lev], P.direct, B.child.as<int2>(), B.sp.as<double2>(), B.m64.as<Mom64>(),
+                       B.geo.as<BhGeo>(), B.mom.as<BhMom>());
     FLK_HIP(hipGetLastError());
   }
-  FLK_HIP(hipStreamSynchronize(s));
-  for (int32_t k = 0; k < nc; ++k) F.root[cs[k]] = k;  // roots are nodes 0..nc-1
+}
+
+void forest_from_plan(flk_ctx *ctx, const ForestPlan &P, Forest &F) {
+  F.root.assign(ctx->curves.ncurves, -1);
+  F.nnodes = P.total;
+  F.depth = (int32_t)P.level_base.size();
+  if (P.cs.empty()) F.depth = 0;
+  for (size_t k = 0; k < P.cs.size(); ++k) F.root[P.cs[k]] = (int32_t)k;  // roots are nodes 0..nc-1
+}
+
+void build_forest(flk_ctx *ctx, const std::vector<int32_t> &curves_in, int leaf, Forest &F) {
+  ForestPlan P;
+  plan_forest(ctx, curves_in, leaf, P);
+  enqueue_forest(ctx, P, ctx->stream);
+  FLK_HIP(hipStreamSynchronize(ctx->stream));
+  forest_from_plan(ctx, P, F);
 }
 
 // Items per thread of the level and leaf kernels: the tuned value, or by
@@ -1305,7 +1454,7 @@ void traverse(flk_ctx *ctx, Forest &F, const std::vector<int32_t> &ra, const std
   std::vector<Level> stack;
   size_t used = 0;  // level buffers handed out from B.levels
   auto level_buf = [&](int64_t items) -> DevBuf * {
-    if (B.levels.size() <= used) B.levels.push_back(std::make_unique<DevBuf>());
+    if (B.levels.size() <= used) B.add_level();
     DevBuf *b = B.levels[used++].get();
     b->reserve(sizeof(BhItem) * std::max<int64_t>(items, 1));
     return b;
@@ -1432,116 +1581,106 @@ int resident_grid(flk_ctx *ctx, const void *kernel) {
   return nb * std::max(1, ctx->num_cus);
 }
 
-// One traversal pass with every level enqueued up front: level d reads its
-// item count from the counter that level d - 1 incremented, on a persistent
-// grid whose blocks stride over the level's chunks, so no count returns to
-// the host until the pass has ended (one read-back per pass).  Every split
-// deepens one of the two trees, so a pass has fewer than 2 * depth + 1
-// levels; launched levels past the last nonempty one see a count of 0 and
-// exit at once.  Both level buffers and the leaf list hold `cap` items.  A
-// level or leaf list beyond that, or items left after the last launched
-// level, make the pass return false: its sums are then incomplete, and the
-// caller clears them and re-runs the pairs on the host-driven schedule.
-bool traverse_device(flk_ctx *ctx, Forest &F, const std::vector<int32_t> &ra, const std::vector<int32_t> &rb,
-                     const std::vector<float> &beta2, float eps, double *d_sum, double *d_err,
-                     unsigned long long stats[4], bool use_mlp, float log_tol) {
-  hipStream_t s = ctx->stream;
+// Device schedule of a batch's traversal passes.  A pass enqueues every
+// level at once: level d reads its item count from the counter that level
+// d - 1 incremented, on a persistent grid whose blocks stride over the
+// level's chunks, so no count returns to the host until the batch has ended.
+// Every split deepens one of the two trees, so a pass has fewer than
+// 2 * depth + 1 levels; launched levels past the last nonempty one see a
+// count of 0 and exit at once.  Both level buffers and the leaf list hold
+// `cap` items.  A level or leaf list beyond that, or items left after the
+// last launched level, mark the pass as overflowed: its sums are incomplete,
+// and the batch is re-run on the host-driven schedule.
+struct PassShape {
+  int64_t cap = 0;
+  int D = 0, nslots = 0, ipt = 4;
+  unsigned grid_lev = 1, grid_leaf = 1;
+};
+
+// Counter words of the host-driven schedule (traverse); a batch's re-run
+// count lives in the 8 words after them.
+constexpr int kHostCounterWords = 4 + 2 * 1024;
+
+// Sizes the device schedule of a batch of nb pairs whose deepest tree has
+// `depth` node levels and reserves its buffers; false when it cannot hold
+// the batch (the host-driven schedule runs it instead).
+bool plan_device_passes(flk_ctx *ctx, int64_t nb, int depth, bool use_mlp, PassShape &P) {
   BhScratch &B = ctx->bh;
-  const int64_t np = (int64_t)ra.size();
-  if (np == 0) return true;
-  constexpr int kMaxLevels = 1000;  // counter slots: 4 + 2 * (levels + 1) <= 4 + 2 * 1024
-  const int D = 2 * F.depth + 1;
-  if (D > kMaxLevels) return false;
+  constexpr int kMaxLevels = 1000;
+  P.D = 2 * depth + 1;
+  if (P.D > kMaxLevels) return false;
   if (B.dev_cap == 0) {
-    // two level buffers and the leaf list (16 B per item) and the predictions
-    // (4 B), within a tenth of the free memory, at most 2^26 items each
+    // two level buffers and the leaf list (16 B per item), within a tenth of
+    // the free memory, at most 2^26 items each
     size_t free_b = 0, total_b = 0;
     FLK_HIP(hipMemGetInfo(&free_b, &total_b));
     int64_t cap = int64_t(1) << 26;
     while (cap > (int64_t(1) << 20) && (double)cap * 52.0 > 0.1 * (double)free_b) cap >>= 1;
     B.dev_cap = cap;
   }
-  const int64_t cap = ctx->tune.bh_level_capacity > 0 ? (int64_t)ctx->tune.bh_level_capacity : B.dev_cap;
-  if (np > cap) return false;
-  const int ipt = items_per_thread(ctx, use_mlp);
-  while (B.levels.size() < 2) B.levels.push_back(std::make_unique<DevBuf>());
-  B.levels[0]->reserve(sizeof(BhItem) * cap);
-  B.levels[1]->reserve(sizeof(BhItem) * cap);
-  B.leaves.reserve(sizeof(BhItem) * cap);
-  if (use_mlp) mlp_upload(ctx);
-  const float *mlp_params = use_mlp ? ctx->mlp.d_params.as<float>() : nullptr;
-  const float *mlp_norm = use_mlp ? ctx->mlp.d_norm.as<float>() : nullptr;
-  B.roots_a.reserve(sizeof(int32_t) * np);
-  B.roots_b.reserve(sizeof(int32_t) * np);
-  B.beta2.reserve(sizeof(float) * np);
-  const int nslots = 4 + 2 * (D + 1);
-  B.cnt.reserve(sizeof(unsigned long long) * std::max(nslots, 4 + 2 * 1024));
-  if (B.dev_counts.size() < (size_t)nslots) B.dev_counts.resize(nslots);
-  FLK_HIP(hipMemcpyAsync(B.roots_a.ptr, ra.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice, s));
-  FLK_HIP(hipMemcpyAsync(B.roots_b.ptr, rb.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice, s));
-  FLK_HIP(hipMemcpyAsync(B.beta2.ptr, beta2.data(), sizeof(float) * np, hipMemcpyHostToDevice, s));
-  // [2] leaf segment pairs, [3] network evaluations, [4 + 2d] items of level
-  // d, [5] leaf items
-  unsigned long long *cnt = B.cnt.as<unsigned long long>();
-  hipLaunchKernelGGL(bh_pass_init, dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, s, cnt, nslots, 4,
-                     (unsigned long long)np);
-  BhItem *buf[2] = {B.levels[0]->as<BhItem>(), B.levels[1]->as<BhItem>()};
-  hipLaunchKernelGGL(bh_roots, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, B.roots_a.as<int32_t>(),
-                     B.roots_b.as<int32_t>(), np, buf[0]);
-  FLK_HIP(hipGetLastError());
-  const LevelKernel lev = level_kernel(ipt, use_mlp);
-  auto leafk = ipt == 2 ? bh_leaf_pairs<2> : ipt == 8 ? bh_leaf_pairs<8> : bh_leaf_pairs<4>;
-  const int64_t items_per_block = (int64_t)ipt * kTravBlock;
-  const unsigned grid_lev = (unsigned)std::min<int64_t>(resident_grid(ctx, (const void *)lev),
-                                                        (cap + items_per_block - 1) / items_per_block);
-  const unsigned grid_leaf = (unsigned)std::min<int64_t>(resident_grid(ctx, (const void *)leafk),
-                                                         (cap + items_per_block - 1) / items_per_block);
-  for (int d = 0; d < D; ++d) {
-    const BhItem *in = buf[d & 1];
-    const unsigned long long *n_dev = cnt + 4 + 2 * d;
-    hipLaunchKernelGGL(lev, dim3(grid_lev), dim3(kTravBlock), 0, s, in, cap, n_dev, F.geo.as<BhGeo>(),
-                       F.mom.as<BhMom>(), (const float *)B.beta2.as<float>(), eps, mlp_params, mlp_norm, log_tol,
-                       cnt + 3, buf[(d + 1) & 1], cnt + 4 + 2 * (d + 1), cap, B.leaves.as<BhItem>(), cnt + 5, cap,
-                       d_sum, d_err);
-    FLK_HIP(hipGetLastError());
-  }
-  const Curves &cv = ctx->curves;
-  hipLaunchKernelGGL(leafk, dim3(grid_leaf), dim3(kTravBlock), 0, s, B.leaves.as<BhItem>(), cap,
-                     (const unsigned long long *)(cnt + 5), F.geo.as<BhGeo>(), F.order.as<uint32_t>(),
-                     cv.hi.as<float4>(), cv.lo.as<float4>(), cv.ed.as<float4>(), d_sum, d_err, cnt);
-  FLK_HIP(hipGetLastError());
-  unsigned long long *h = B.dev_counts.data();
-  FLK_HIP(hipMemcpyAsync(h, cnt, sizeof(unsigned long long) * nslots, hipMemcpyDeviceToHost, s));
-  FLK_HIP(hipStreamSynchronize(s));
-  // every item is exactly one of: leaf pair, far field, split (two children)
-  const unsigned long long nleaf = h[5];
-  if ((int64_t)nleaf > cap || h[4 + 2 * D] != 0) return false;
-  unsigned long long visits = 0, split_children = 0;
-  for (int d = 0; d < D; ++d) {
-    if ((int64_t)h[4 + 2 * d] > cap || (int64_t)h[4 + 2 * (d + 1)] > cap) return false;
-    visits += h[4 + 2 * d];
-    split_children += h[4 + 2 * (d + 1)];
-  }
-  stats[0] += visits;
-  stats[1] += visits - split_children / 2 - nleaf;
-  stats[2] += h[2];
-  stats[3] += h[3];
+  P.cap = ctx->tune.bh_level_capacity > 0 ? (int64_t)ctx->tune.bh_level_capacity : B.dev_cap;
+  if (nb > P.cap) return false;
+  P.ipt = items_per_thread(ctx, use_mlp);
+  P.nslots = 4 + 2 * (P.D + 1);
+  while (B.levels.size() < 2) B.add_level();
+  B.levels[0]->reserve(sizeof(BhItem) * P.cap);
+  B.levels[1]->reserve(sizeof(BhItem) * P.cap);
+  B.leaves.reserve(sizeof(BhItem) * P.cap);
+  const LevelKernel lev = level_kernel(P.ipt, use_mlp);
+  auto leafk = P.ipt == 2 ? bh_leaf_pairs<2> : P.ipt == 8 ? bh_leaf_pairs<8> : bh_leaf_pairs<4>;
+  const int64_t items_per_block = (int64_t)P.ipt * kTravBlock;
+  const int64_t chunks = (P.cap + items_per_block - 1) / items_per_block;
+  P.grid_lev = (unsigned)std::min<int64_t>(resident_grid(ctx, (const void *)lev), chunks);
+  P.grid_leaf = (unsigned)std::min<int64_t>(resident_grid(ctx, (const void *)leafk), chunks);
   return true;
 }
 
-// A pass on the device schedule, or on the host-driven chunked one when
-// per-launch timing is asked for or the device schedule's buffers overflow.
-void traverse_pass(flk_ctx *ctx, Forest &F, const std::vector<int32_t> &ra, const std::vector<int32_t> &rb,
-                   const std::vector<float> &beta2, float eps, double *d_sum, double *d_err,
-                   unsigned long long stats[4], bool use_mlp, float log_tol, bool timing) {
-  if (!timing && traverse_device(ctx, F, ra, rb, beta2, eps, d_sum, d_err, stats, use_mlp, log_tol)) return;
-  if (!timing) {  // the incomplete pass's sums are discarded
-    const size_t np = ra.size();
-    FLK_HIP(hipMemsetAsync(d_sum, 0, sizeof(double) * np, ctx->stream));
-    FLK_HIP(hipMemsetAsync(d_err, 0, sizeof(double) * np, ctx->stream));
+// Enqueues one pass on the device schedule: counters at cnt (the caller's
+// kernel has cleared them and set the root count at cnt[4]), level-0 items at
+// items0, per-pair beta^2 and sums indexed by the items' pair field.
+void enqueue_pass(flk_ctx *ctx, const PassShape &P, const BhItem *items0, const float *beta2, float eps,
+                  bool use_mlp, float log_tol, unsigned long long *cnt, double *d_sum, double *d_err,
+                  hipStream_t s) {
+  BhScratch &B = ctx->bh;
+  const Curves &cv = ctx->curves;
+  BhItem *buf[2] = {B.levels[0]->as<BhItem>(), B.levels[1]->as<BhItem>()};
+  const float *mlp_params = use_mlp ? ctx->mlp.d_params.as<float>() : nullptr;
+  const float *mlp_norm = use_mlp ? ctx->mlp.d_norm.as<float>() : nullptr;
+  const LevelKernel lev = level_kernel(P.ipt, use_mlp);
+  auto leafk = P.ipt == 2 ? bh_leaf_pairs<2> : P.ipt == 8 ? bh_leaf_pairs<8> : bh_leaf_pairs<4>;
+  // [2] leaf segment pairs, [3] network evaluations, [4 + 2d] items of level
+  // d, [5] leaf items
+  for (int d = 0; d < P.D; ++d) {
+    const BhItem *in = d == 0 ? items0 : buf[(d - 1) & 1];
+    hipLaunchKernelGGL(lev, dim3(P.grid_lev), dim3(kTravBlock), 0, s, in, P.cap,
+                       (const unsigned long long *)(cnt + 4 + 2 * d), B.geo.as<BhGeo>(), B.mom.as<BhMom>(), beta2,
+                       eps, mlp_params, mlp_norm, log_tol, cnt + 3, buf[d & 1], cnt + 4 + 2 * (d + 1), P.cap,
+                       B.leaves.as<BhItem>(), cnt + 5, P.cap, d_sum, d_err);
+    FLK_HIP(hipGetLastError());
   }
-  ctx->stats.bh_host_passes += 1;
-  traverse(ctx, F, ra, rb, beta2, eps, d_sum, d_err, stats, use_mlp, log_tol, timing);
+  hipLaunchKernelGGL(leafk, dim3(P.grid_leaf), dim3(kTravBlock), 0, s, B.leaves.as<BhItem>(), P.cap,
+                     (const unsigned long long *)(cnt + 5), B.geo.as<BhGeo>(), B.order.as<uint32_t>(),
+                     cv.hi.as<float4>(), cv.lo.as<float4>(), cv.ed.as<float4>(), d_sum, d_err, cnt);
+  FLK_HIP(hipGetLastError());
+}
+
+// Adds a device-scheduled pass's visits, far fields, leaf segment pairs and
+// network evaluations (from its counters h) to st; false when it overflowed.
+// Every item is exactly one of: leaf pair, far field, split (two children).
+bool pass_counts(const unsigned long long *h, const PassShape &P, unsigned long long st[4]) {
+  const unsigned long long nleaf = h[5];
+  if ((int64_t)nleaf > P.cap || h[4 + 2 * P.D] != 0) return false;
+  unsigned long long visits = 0, split_children = 0;
+  for (int d = 0; d < P.D; ++d) {
+    if ((int64_t)h[4 + 2 * d] > P.cap || (int64_t)h[4 + 2 * (d + 1)] > P.cap) return false;
+    visits += h[4 + 2 * d];
+    split_children += h[4 + 2 * (d + 1)];
+  }
+  st[0] += visits;
+  st[1] += visits - split_children / 2 - nleaf;
+  st[2] += h[2];
+  st[3] += h[3];
+  return true;
 }
 
 }  // namespace
@@ -1658,101 +1797,274 @@ void tree_export(flk_ctx *ctx, int32_t curve, int32_t leaf, std::vector<flk_tree
 
 namespace {
 
-// Trees, traversal and the adaptive-beta / learned-criterion re-run for one
-// batch of Barnes–Hut pairs (indices into `pairs`); results into lk/err/flags.
-void bh_batch(flk_ctx *ctx, const int32_t *pairs, const std::vector<int64_t> &bh_idx, const flk_params &prm,
-              std::vector<double> &lk, std::vector<double> &err, std::vector<uint32_t> &flags) {
+// One batch of Barnes–Hut pairs (indices into the call's pair list): its
+// forest plan, root nodes, first-pass beta^2, the device schedule's shape,
+// and the results and counters the batch's read-back fills.
+struct BatchPlan {
+  std::vector<int64_t> idx;
+  ForestPlan forest;
+  std::vector<int32_t> ra, rb;
+  std::vector<float> beta2;
+  PassShape shape;
+  bool device = false;  // the device schedule holds it
+  std::vector<double> v, e;
+  std::vector<uint32_t> f;
+  std::vector<unsigned long long> counts;  // both passes' counters, nslots each
+  // In a graph: ra, rb and beta2 uploaded once (dsrc), and the read-back into
+  // pinned memory (res), copied into v, e, f and counts after the launch.
+  DevBuf dconst;
+  std::vector<const void *> dsrc;
+  HostBuf res;
+  double *hv = nullptr, *he = nullptr;
+  uint32_t *hf = nullptr;
+  unsigned long long *hc = nullptr;
+  // where the read-back lands: the vectors, or res
+  void land_in_vectors() {
+    hv = v.data();
+    he = e.data();
+    hf = f.data();
+    hc = counts.data();
+  }
+  void land_pinned(hipStream_t s) {
+    const size_t nb = idx.size(), nc = counts.size();
+    res.reserve(16 * nb + 4 * nb + 8 * nc + 64);
+    unsigned char *p = res.as<unsigned char>();
+    hv = reinterpret_cast<double *>(p);
+    he = hv + nb;
+    hc = reinterpret_cast<unsigned long long *>(he + nb);
+    hf = reinterpret_cast<uint32_t *>(hc + nc);
+    dsrc = upload_pack(dconst, {{ra.data(), 4 * nb}, {rb.data(), 4 * nb}, {beta2.data(), 4 * nb}}, s);
+    forest.dsrc = upload_pack(forest.dconst,
+                              {{forest.seg_off.data(), 8 * forest.seg_off.size()},
+                               {forest.vtx_off.data(), 8 * forest.vtx_off.size()},
+                               {forest.cbox.data(), 8 * forest.cbox.size()},
+                               {forest.roots.data(), sizeof(BhRange) * forest.roots.size()}},
+                              s);
+  }
+  void collect() {  // pinned read-back into the vectors
+    if (hv == v.data()) return;
+    std::copy(hv, hv + v.size(), v.begin());
+    std::copy(he, he + e.size(), e.begin());
+    std::copy(hf, hf + f.size(), f.begin());
+    std::copy(hc, hc + counts.size(), counts.begin());
+  }
+};
+
+void plan_batch(flk_ctx *ctx, const int32_t *pairs, std::vector<int64_t> idx, const flk_params &prm,
+                BatchPlan &bp) {
   Curves &cv = ctx->curves;
-  hipStream_t s = ctx->stream;
   const bool use_mlp = prm.criterion == FLK_CRIT_MLP;
-  const int leaf = std::max(1, prm.leaf_size);
+  bp.idx = std::move(idx);
   std::vector<char> used(cv.ncurves, 0);
-  for (int64_t k : bh_idx) used[pairs[2 * k]] = used[pairs[2 * k + 1]] = 1;
+  for (int64_t k : bp.idx) used[pairs[2 * k]] = used[pairs[2 * k + 1]] = 1;
   std::vector<int32_t> cs;
   for (int32_t c = 0; c < cv.ncurves; ++c)
     if (used[c]) cs.push_back(c);
-  Forest F(ctx->bh);
-  auto t0 = std::chrono::steady_clock::now();
-  build_forest(ctx, cs, leaf, F);
-  ctx->stats.ms_bh_build +=
-      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-  const int64_t nb = (int64_t)bh_idx.size();
-  std::vector<int32_t> ra(nb), rb(nb);
-  std::vector<float> beta2(nb);
+  plan_forest(ctx, cs, std::max(1, prm.leaf_size), bp.forest);
+  std::vector<int32_t> root(cv.ncurves, -1);
+  for (size_t k = 0; k < bp.forest.cs.size(); ++k) root[bp.forest.cs[k]] = (int32_t)k;
+  const int64_t nb = (int64_t)bp.idx.size();
+  bp.ra.resize(nb);
+  bp.rb.resize(nb);
+  bp.beta2.assign(nb, (float)(prm.beta_init * prm.beta_init));
   for (int64_t t = 0; t < nb; ++t) {
-    ra[t] = F.root[pairs[2 * bh_idx[t]]];
-    rb[t] = F.root[pairs[2 * bh_idx[t] + 1]];
-    beta2[t] = (float)(prm.beta_init * prm.beta_init);
+    bp.ra[t] = root[pairs[2 * bp.idx[t]]];
+    bp.rb[t] = root[pairs[2 * bp.idx[t] + 1]];
   }
+  bp.device = plan_device_passes(ctx, nb, (int)bp.forest.level_base.size(), use_mlp, bp.shape);
+  BhScratch &B = ctx->bh;
+  B.roots_a.reserve(sizeof(int32_t) * nb);
+  B.roots_b.reserve(sizeof(int32_t) * nb);
+  B.beta2.reserve(sizeof(float) * 2 * nb);
+  B.items0.reserve(sizeof(BhItem) * nb);
+  B.redo.reserve(sizeof(int32_t) * nb);
+  B.sum.reserve(sizeof(double) * 2 * nb);
+  B.err.reserve(sizeof(double) * 2 * nb);
+  B.outv.reserve(sizeof(double) * nb);
+  B.oute.reserve(sizeof(double) * nb);
+  B.outf.reserve(sizeof(uint32_t) * nb);
+  B.cnt.reserve(sizeof(unsigned long long) * std::max(2 * bp.shape.nslots, kHostCounterWords + 8));
+  if (use_mlp) mlp_upload(ctx);
+  bp.v.assign(nb, 0.0);
+  bp.e.assign(nb, 0.0);
+  bp.f.assign(nb, 0u);
+  bp.counts.assign(2 * (size_t)bp.shape.nslots, 0ull);
+  bp.land_in_vectors();
+}
+
+// The batch's merge of both passes into per-pair results, and their read-back.
+void enqueue_merge(flk_ctx *ctx, BatchPlan &bp, const flk_params &prm, const unsigned long long *n_redo,
+                   hipStream_t s) {
+  BhScratch &B = ctx->bh;
+  const int64_t nb = (int64_t)bp.idx.size();
+  const int mlp = prm.criterion == FLK_CRIT_MLP ? 1 : 0;
+  const unsigned g = (unsigned)((nb + 255) / 256);
+  hipLaunchKernelGGL(bh_merge_first, dim3(g), dim3(256), 0, s, nb, (const double *)B.sum.as<double>(),
+                     (const double *)B.err.as<double>(), mlp, prm.err_scale, B.outv.as<double>(),
+                     B.oute.as<double>(), B.outf.as<uint32_t>());
+  hipLaunchKernelGGL(bh_merge_redo, dim3(g), dim3(256), 0, s, nb, n_redo, (const int32_t *)B.redo.as<int32_t>(),
+                     (const double *)(B.sum.as<double>() + nb), (const double *)(B.err.as<double>() + nb), mlp,
+                     prm.err_scale, B.outv.as<double>(), B.oute.as<double>(), B.outf.as<uint32_t>());
+  FLK_HIP(hipGetLastError());
+  FLK_HIP(hipMemcpyAsync(bp.hv, B.outv.ptr, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+  FLK_HIP(hipMemcpyAsync(bp.he, B.oute.ptr, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+  FLK_HIP(hipMemcpyAsync(bp.hf, B.outf.ptr, sizeof(uint32_t) * nb, hipMemcpyDeviceToHost, s));
+}
+
+void enqueue_adapt(flk_ctx *ctx, int64_t nb, const flk_params &prm, unsigned long long *cnt2, int nslots2,
+                   hipStream_t s) {
+  BhScratch &B = ctx->bh;
+  hipLaunchKernelGGL(bh_adapt, dim3(1), dim3(256), 0, s, nb, (const int32_t *)B.roots_a.as<int32_t>(),
+                     (const int32_t *)B.roots_b.as<int32_t>(), (const double *)B.err.as<double>(),
+                     prm.criterion == FLK_CRIT_MLP ? 1 : 0, prm.err_scale, prm.target_error, prm.beta_init,
+                     prm.beta_limit, B.items0.as<BhItem>(), B.beta2.as<float>() + nb, B.redo.as<int32_t>(), cnt2,
+                     nslots2, 4);
+  FLK_HIP(hipGetLastError());
+}
+
+// Traversal of a planned batch on the device schedule, after its forest:
+// first pass, the re-run decision (bh_adapt), second pass over the re-run
+// list, the merge, and the read-back of results and counters.  Kernels and
+// copies only, so it can be captured.
+void enqueue_traversal(flk_ctx *ctx, BatchPlan &bp, const flk_params &prm, hipStream_t s) {
+  BhScratch &B = ctx->bh;
+  const Curves &cv = ctx->curves;
+  const bool use_mlp = prm.criterion == FLK_CRIT_MLP;
+  const int64_t nb = (int64_t)bp.idx.size();
+  const PassShape &P = bp.shape;
   const float eps = (float)(prm.eps_sq * cv.scale * cv.scale);
-  // per-pair sums: [0, nb) first pass, [nb, 2 nb) re-runs
-  ctx->bh.sum.reserve(sizeof(double) * 2 * nb);
-  ctx->bh.err.reserve(sizeof(double) * 2 * nb);
-  double *d_sum = ctx->bh.sum.as<double>(), *d_err = ctx->bh.err.as<double>();
-  FLK_HIP(hipMemsetAsync(d_sum, 0, sizeof(double) * 2 * nb, s));
-  FLK_HIP(hipMemsetAsync(d_err, 0, sizeof(double) * 2 * nb, s));
+  const float log_tol = (float)prm.mlp_log_tol;
+  double *sum = B.sum.as<double>(), *err = B.err.as<double>();
+  hipLaunchKernelGGL(bh_zero_f64, dim3((unsigned)((2 * nb + 255) / 256)), dim3(256), 0, s, sum, 2 * nb);
+  hipLaunchKernelGGL(bh_zero_f64, dim3((unsigned)((2 * nb + 255) / 256)), dim3(256), 0, s, err, 2 * nb);
+  put(B.roots_a.ptr, bp.ra.data(), bp.dsrc, 0, sizeof(int32_t) * nb, s);
+  put(B.roots_b.ptr, bp.rb.data(), bp.dsrc, 1, sizeof(int32_t) * nb, s);
+  put(B.beta2.ptr, bp.beta2.data(), bp.dsrc, 2, sizeof(float) * nb, s);
+  unsigned long long *c1 = B.cnt.as<unsigned long long>(), *c2 = c1 + P.nslots;
+  hipLaunchKernelGGL(bh_pass_init, dim3((unsigned)((P.nslots + 255) / 256)), dim3(256), 0, s, c1, P.nslots, 4,
+                     (unsigned long long)nb);
+  hipLaunchKernelGGL(bh_roots, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s,
+                     (const int32_t *)B.roots_a.as<int32_t>(), (const int32_t *)B.roots_b.as<int32_t>(), nb,
+                     B.items0.as<BhItem>());
+  FLK_HIP(hipGetLastError());
+  enqueue_pass(ctx, P, B.items0.as<BhItem>(), B.beta2.as<float>(), eps, use_mlp, log_tol, c1, sum, err, s);
+  enqueue_adapt(ctx, nb, prm, c2, P.nslots, s);
+  enqueue_pass(ctx, P, B.items0.as<BhItem>(), B.beta2.as<float>() + nb, eps, use_mlp, log_tol - 1.0f, c2,
+               sum + nb, err + nb, s);
+  enqueue_merge(ctx, bp, prm, c2 + 4, s);
+  FLK_HIP(hipMemcpyAsync(bp.hc, c1, sizeof(unsigned long long) * 2 * P.nslots, hipMemcpyDeviceToHost,
+                         s));
+}
+
+// Results and statistics of a batch whose read-back has arrived; false when
+// a pass overflowed the device schedule (nothing is written then).
+bool finish_batch(flk_ctx *ctx, const BatchPlan &bp, std::vector<double> &lk, std::vector<double> &err,
+                  std::vector<uint32_t> &flags) {
   unsigned long long st[4] = {0, 0, 0, 0};
-  t0 = std::chrono::steady_clock::now();
-  traverse_pass(ctx, F, ra, rb, beta2, eps, d_sum, d_err, st, use_mlp, (float)prm.mlp_log_tol, prm.timing != 0);
-  std::vector<double> v(nb), e(nb);
-  FLK_HIP(hipMemcpyAsync(v.data(), d_sum, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
-  FLK_HIP(hipMemcpyAsync(e.data(), d_err, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
-  FLK_HIP(hipStreamSynchronize(s));
-  // Adaptive beta (reference linking_number_certificate.cpp:165-184): scale
-  // the estimate, target beta = min(beta_init (E / target)^(1/4), limit).
-  std::vector<int64_t> redo;
-  std::vector<int32_t> ra2, rb2;
-  std::vector<float> b2;
-  for (int64_t t = 0; t < nb && !use_mlp; ++t) {
-    const double E = e[t] * prm.err_scale;
-    const double target = std::min(std::pow(E / prm.target_error, 0.25) * prm.beta_init, prm.beta_limit);
-    e[t] = E;
-    if (target > prm.beta_init) {
-      redo.push_back(t);
-      ra2.push_back(ra[t]);
-      rb2.push_back(rb[t]);
-      b2.push_back((float)(target * target));
-    }
+  const unsigned long long *h = bp.counts.data();
+  if (!pass_counts(h, bp.shape, st) || !pass_counts(h + bp.shape.nslots, bp.shape, st)) return false;
+  for (size_t t = 0; t < bp.idx.size(); ++t) {
+    lk[bp.idx[t]] = bp.v[t];
+    err[bp.idx[t]] = bp.e[t];
+    flags[bp.idx[t]] |= bp.f[t];
   }
-  // Learned criterion: the root-sum-square of the predicted errors is the
-  // estimate (it tracks the observed error within a few times, where the
-  // reference's summed Eq. 14 bound is scaled by 0.05); a pair whose estimate
-  // exceeds a tenth of the target is re-run one decade stricter.
-  for (int64_t t = 0; t < nb && use_mlp; ++t) {
-    e[t] = std::sqrt(e[t]);
-    if (e[t] > 0.1 * prm.target_error) {
-      redo.push_back(t);
-      ra2.push_back(ra[t]);
-      rb2.push_back(rb[t]);
-      b2.push_back((float)(prm.beta_init * prm.beta_init));
-    }
-  }
-  if (!redo.empty()) {
-    const int64_t nr = (int64_t)redo.size();
-    double *d_sum2 = d_sum + nb, *d_err2 = d_err + nb;
-    traverse_pass(ctx, F, ra2, rb2, b2, eps, d_sum2, d_err2, st, use_mlp, (float)prm.mlp_log_tol - 1.0f,
-             prm.timing != 0);
-    std::vector<double> v2(nr), e2(nr);
-    FLK_HIP(hipMemcpyAsync(v2.data(), d_sum2, sizeof(double) * nr, hipMemcpyDeviceToHost, s));
-    FLK_HIP(hipMemcpyAsync(e2.data(), d_err2, sizeof(double) * nr, hipMemcpyDeviceToHost, s));
-    FLK_HIP(hipStreamSynchronize(s));
-    for (int64_t t = 0; t < nr; ++t) {
-      v[redo[t]] = v2[t];
-      e[redo[t]] = use_mlp ? std::sqrt(e2[t]) : e2[t] * prm.err_scale;
-      flags[bh_idx[redo[t]]] |= FLK_PAIR_RETRY;
-    }
-    ctx->stats.retries += nr;
-  }
-  ctx->stats.ms_bh_traverse +=
-      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  ctx->stats.retries += (int64_t)h[bp.shape.nslots + 4];
   ctx->stats.bh_visits += (int64_t)st[0];
   ctx->stats.bh_far += (int64_t)st[1];
   ctx->stats.bh_leaf_pairs += (int64_t)st[2];
   ctx->stats.mlp_evals += (int64_t)st[3];
-  for (int64_t t = 0; t < nb; ++t) {
-    lk[bh_idx[t]] = v[t];
-    err[bh_idx[t]] = e[t];
+  return true;
+}
+
+// A batch on the host-driven schedule: per-launch timing, or a batch the
+// device schedule cannot hold or that overflowed it.  The same re-run
+// decision (bh_adapt) and merge; the re-run list comes back to the host.
+void bh_batch_host(flk_ctx *ctx, const int32_t *pairs, const std::vector<int64_t> &idx, const flk_params &prm,
+                   std::vector<double> &lk, std::vector<double> &err, std::vector<uint32_t> &flags) {
+  hipStream_t s = ctx->stream;
+  const bool use_mlp = prm.criterion == FLK_CRIT_MLP;
+  BatchPlan bp;
+  auto t0 = std::chrono::steady_clock::now();
+  plan_batch(ctx, pairs, idx, prm, bp);
+  enqueue_forest(ctx, bp.forest, s);
+  FLK_HIP(hipStreamSynchronize(s));
+  ctx->stats.ms_bh_build +=
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  t0 = std::chrono::steady_clock::now();
+  Forest F(ctx->bh);
+  forest_from_plan(ctx, bp.forest, F);
+  BhScratch &B = ctx->bh;
+  const int64_t nb = (int64_t)bp.idx.size();
+  const float eps = (float)(prm.eps_sq * ctx->curves.scale * ctx->curves.scale);
+  double *sum = B.sum.as<double>(), *err_d = B.err.as<double>();
+  hipLaunchKernelGGL(bh_zero_f64, dim3((unsigned)((2 * nb + 255) / 256)), dim3(256), 0, s, sum, 2 * nb);
+  hipLaunchKernelGGL(bh_zero_f64, dim3((unsigned)((2 * nb + 255) / 256)), dim3(256), 0, s, err_d, 2 * nb);
+  unsigned long long st[4] = {0, 0, 0, 0};
+  traverse(ctx, F, bp.ra, bp.rb, bp.beta2, eps, sum, err_d, st, use_mlp, (float)prm.mlp_log_tol, prm.timing != 0);
+  unsigned long long *c2 = B.cnt.as<unsigned long long>() + kHostCounterWords;
+  enqueue_adapt(ctx, nb, prm, c2, 8, s);
+  unsigned long long nr = 0;
+  FLK_HIP(hipMemcpyAsync(&nr, c2 + 4, sizeof nr, hipMemcpyDeviceToHost, s));
+  FLK_HIP(hipStreamSynchronize(s));
+  int passes = 1;
+  if (nr > 0) {
+    std::vector<BhItem> it(nr);
+    std::vector<float> b2(nr);
+    FLK_HIP(hipMemcpyAsync(it.data(), B.items0.ptr, sizeof(BhItem) * nr, hipMemcpyDeviceToHost, s));
+    FLK_HIP(hipMemcpyAsync(b2.data(), B.beta2.as<float>() + nb, sizeof(float) * nr, hipMemcpyDeviceToHost, s));
+    FLK_HIP(hipStreamSynchronize(s));
+    std::vector<int32_t> ra2(nr), rb2(nr);
+    for (size_t k = 0; k < nr; ++k) {
+      ra2[k] = it[k].a;
+      rb2[k] = it[k].b;
+    }
+    traverse(ctx, F, ra2, rb2, b2, eps, sum + nb, err_d + nb, st, use_mlp, (float)prm.mlp_log_tol - 1.0f,
+             prm.timing != 0);
+    passes = 2;
   }
+  enqueue_merge(ctx, bp, prm, c2 + 4, s);
+  FLK_HIP(hipStreamSynchronize(s));
+  for (size_t t = 0; t < bp.idx.size(); ++t) {
+    lk[bp.idx[t]] = bp.v[t];
+    err[bp.idx[t]] = bp.e[t];
+    flags[bp.idx[t]] |= bp.f[t];
+  }
+  ctx->stats.retries += (int64_t)nr;
+  ctx->stats.bh_host_passes += passes;
+  ctx->stats.bh_visits += (int64_t)st[0];
+  ctx->stats.bh_far += (int64_t)st[1];
+  ctx->stats.bh_leaf_pairs += (int64_t)st[2];
+  ctx->stats.mlp_evals += (int64_t)st[3];
+  ctx->stats.ms_bh_traverse +=
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Trees, traversal and the adaptive-beta / learned-criterion re-run for one
+// batch of Barnes–Hut pairs, launched eagerly; results into lk/err/flags.
+void bh_batch(flk_ctx *ctx, const int32_t *pairs, const std::vector<int64_t> &idx, const flk_params &prm,
+              std::vector<double> &lk, std::vector<double> &err, std::vector<uint32_t> &flags) {
+  if (prm.timing) {
+    bh_batch_host(ctx, pairs, idx, prm, lk, err, flags);
+    return;
+  }
+  hipStream_t s = ctx->stream;
+  BatchPlan bp;
+  auto t0 = std::chrono::steady_clock::now();
+  plan_batch(ctx, pairs, idx, prm, bp);
+  if (!bp.device) {
+    bh_batch_host(ctx, pairs, idx, prm, lk, err, flags);
+    return;
+  }
+  enqueue_forest(ctx, bp.forest, s);
+  FLK_HIP(hipStreamSynchronize(s));
+  ctx->stats.ms_bh_build +=
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  t0 = std::chrono::steady_clock::now();
+  enqueue_traversal(ctx, bp, prm, s);
+  FLK_HIP(hipStreamSynchronize(s));
+  ctx->stats.ms_bh_traverse +=
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (!finish_batch(ctx, bp, lk, err, flags)) bh_batch_host(ctx, pairs, idx, prm, lk, err, flags);
 }
 
 // Device bytes the trees and their build records take per segment: two
@@ -1761,6 +2073,173 @@ void bh_batch(flk_ctx *ctx, const int32_t *pairs, const std::vector<int64_t> &bh
 // ids and order (24), the staged segment endpoints (48), scan flags (8) and
 // sort scratch, with headroom.
 constexpr double kBhBytesPerSegment = 648.0;
+
+// Batches of Barnes–Hut pairs whose curves' trees fit the device: the
+// segment budget is bh_batch_ksegs thousand, or what free memory (plus the
+// scratch this context already holds) allows when a curve set first runs
+// Barnes–Hut -- kept for later calls on the same curves, so repeated
+// certificates form the same batches and reuse the same scratch.  A curve
+// shared by two batches is built in both.
+std::vector<std::vector<int64_t>> form_batches(flk_ctx *ctx, const int32_t *pairs, const std::vector<int64_t> &bh_idx,
+                                               const flk_params &prm) {
+  Curves &cv = ctx->curves;
+  int64_t budget;
+  if (prm.bh_batch_ksegs > 0) {
+    budget = (int64_t)prm.bh_batch_ksegs * 1000;
+  } else {
+    if (cv.bh_budget <= 0) {
+      size_t free_b = 0, total_b = 0;
+      FLK_HIP(hipMemGetInfo(&free_b, &total_b));
+      cv.bh_budget = (int64_t)(0.6 * (double)(free_b + ctx->bh.bytes()) / kBhBytesPerSegment);
+    }
+    budget = cv.bh_budget;
+  }
+  budget = std::max<int64_t>(budget, 1);
+  std::vector<std::vector<int64_t>> out;
+  std::vector<char> in_batch(cv.ncurves, 0);
+  std::vector<int32_t> members;
+  std::vector<int64_t> batch;
+  int64_t segs = 0;
+  auto flush = [&]() {
+    if (batch.empty()) return;
+    out.push_back(batch);
+    for (int32_t c : members) in_batch[c] = 0;
+    members.clear();
+    batch.clear();
+    segs = 0;
+  };
+  for (int64_t k : bh_idx) {
+    const int32_t a = pairs[2 * k], b = pairs[2 * k + 1];
+    const int64_t add = (in_batch[a] ? 0 : cv.n[a]) + (in_batch[b] || a == b ? 0 : cv.n[b]);
+    if (!batch.empty() && segs + add > budget) flush();
+    for (int32_t c : {a, b})
+      if (!in_batch[c]) {
+        in_batch[c] = 1;
+        members.push_back(c);
+        segs += cv.n[c];
+      }
+    batch.push_back(k);
+  }
+  flush();
+  return out;
+}
+
+uint64_t hash_bytes(uint64_t h, const void *p, size_t n) {
+  const unsigned char *c = static_cast<const unsigned char *>(p);
+  for (size_t i = 0; i < n; ++i) h = (h ^ c[i]) * 0x100000001b3ULL;
+  return h;
+}
+
+}  // namespace
+
+// A whole Barnes–Hut certificate as one captured graph: the direct-sum batch
+// of the pairs below the threshold (uploads, persistent tile kernel,
+// reduction, read-back) and, per Barnes–Hut batch, the forest build (sort,
+// levels, moments), both traversal passes with the device re-run decision
+// between them, the merge and the read-back.  The device pointers it holds
+// belong to the context's scratch: it is replayed only while no DevBuf has
+// been allocated or freed since its capture (g_devbuf_epoch), and the host
+// arrays its copies read live here.
+struct CertGraph {
+  uint64_t key = 0, epoch = 0;
+  hipGraphExec_t exec = nullptr;
+  std::vector<int64_t> ds_idx;
+  DsGraph ds;
+  std::vector<BatchPlan> batches;
+  ~CertGraph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+  }
+};
+
+namespace {
+
+// Captures (or finds) the certificate graph for these pairs and launches it;
+// false when a batch does not fit the device schedule (the caller then runs
+// the eager path).
+bool bh_run_graph(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, const flk_params &prm,
+                  const std::vector<int64_t> &ds_idx, const std::vector<int32_t> &ds_pairs,
+                  const std::vector<int64_t> &bh_idx, std::vector<double> &lk, std::vector<double> &err,
+                  std::vector<uint32_t> &flags) {
+  hipStream_t s = ctx->stream;
+  const bool use_mlp = prm.criterion == FLK_CRIT_MLP;
+  uint64_t key = hash_bytes(0xcbf29ce484222325ULL, pairs, sizeof(int32_t) * 2 * (size_t)npairs);
+  const double pd[7] = {prm.beta_init, prm.beta_limit, prm.eps_sq, prm.target_error, prm.err_scale,
+                        prm.mlp_log_tol, 0.0};
+  const int64_t pi[8] = {prm.criterion, prm.bh_direct_threshold, prm.leaf_size, prm.fp64, prm.bh_batch_ksegs,
+                         (int64_t)ctx->curves_version, ctx->curves.inexact ? 1 : 0, npairs};
+  key = hash_bytes(key, pd, sizeof pd);
+  key = hash_bytes(key, pi, sizeof pi);
+  key = hash_bytes(key, &ctx->tune, sizeof ctx->tune);
+  CertGraph *G = ctx->cert_graph.get();
+  if (!(G && G->key == key && G->epoch == g_devbuf_epoch.load())) {
+    ctx->cert_graph.reset();
+    auto NG = std::make_shared<CertGraph>();
+    NG->key = key;
+    NG->ds_idx = ds_idx;
+    if (!ds_idx.empty()) ds_capture_prepare(ctx, ds_pairs.data(), (int64_t)ds_idx.size(), prm.fp64 != 0, NG->ds);
+    for (auto &idx : form_batches(ctx, pairs, bh_idx, prm)) {
+      NG->batches.emplace_back();
+      BatchPlan &bp = NG->batches.back();
+      plan_batch(ctx, pairs, std::move(idx), prm, bp);
+      if (!bp.device) return false;
+      bp.land_pinned(s);
+    }
+    FLK_HIP(hipStreamSynchronize(s));
+    hipGraph_t graph = nullptr;
+    FLK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    t_capturing = true;
+    try {
+      if (!ds_idx.empty()) ds_capture_enqueue(ctx, NG->ds, s);
+      for (BatchPlan &bp : NG->batches) {
+        enqueue_forest(ctx, bp.forest, s);
+        enqueue_traversal(ctx, bp, prm, s);
+      }
+    } catch (...) {
+      t_capturing = false;
+      (void)hipStreamEndCapture(s, &graph);
+      if (graph) (void)hipGraphDestroy(graph);
+      throw;
+    }
+    t_capturing = false;
+    FLK_HIP(hipStreamEndCapture(s, &graph));
+    const hipError_t ie = hipGraphInstantiate(&NG->exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) {
+      NG->exec = nullptr;
+      throw Error(FLK_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+    }
+    NG->epoch = g_devbuf_epoch.load();
+    ctx->cert_graph = NG;
+    G = NG.get();
+  } else if (use_mlp) {
+    mlp_upload(ctx);  // the weights as they are now (same buffers)
+  }
+  FLK_HIP(hipEventRecord(ctx->ev0, s));
+  const auto t0 = std::chrono::steady_clock::now();
+  FLK_HIP(hipGraphLaunch(G->exec, s));
+  ctx->stats.ms_graph_launch =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  FLK_HIP(hipEventRecord(ctx->ev1, s));
+  FLK_HIP(hipStreamSynchronize(s));
+  float ms = 0.f;
+  FLK_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  ctx->stats.ms_kernel = ms;
+  ctx->stats.graph_launches += 1;
+  if (!G->ds_idx.empty()) {
+    for (size_t t = 0; t < G->ds_idx.size(); ++t) lk[G->ds_idx[t]] = G->ds.h_out.as<double>()[t];
+    ctx->stats.ds_seg_pairs += G->ds.work;
+    ctx->stats.ds_launches += G->ds.tasks.empty() ? 0 : 1;
+    ctx->stats.ds_waves = G->ds.waves;
+  }
+  // keep the graph alive while batches that overflowed re-run eagerly (their
+  // allocations retire it for the next call)
+  std::shared_ptr<CertGraph> hold = ctx->cert_graph;
+  for (BatchPlan &bp : G->batches) {
+    bp.collect();
+    if (!finish_batch(ctx, bp, lk, err, flags)) bh_batch_host(ctx, pairs, bp.idx, prm, lk, err, flags);
+  }
+  return true;
+}
 
 }  // namespace
 
@@ -1790,6 +2269,11 @@ void bh_run(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, const flk_params
       bh_idx.push_back(k);
     }
   }
+  // Default: the whole certificate (direct-sum part and every Barnes–Hut
+  // batch) as one graph launch.
+  if (prm.graph && !prm.timing && !bh_idx.empty() &&
+      bh_run_graph(ctx, pairs, npairs, prm, ds_idx, ds_pairs, bh_idx, lk, err, flags))
+    return;
   if (!ds_idx.empty()) {
     std::vector<double> v;
     float ms = 0.f;
@@ -1797,50 +2281,7 @@ void bh_run(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, const flk_params
     ctx->stats.ms_ds_kernel += ms;
     for (size_t t = 0; t < ds_idx.size(); ++t) lk[ds_idx[t]] = v[t];
   }
-  if (bh_idx.empty()) return;
-  // Batches of pairs whose curves' trees fit the device: the segment budget
-  // is bh_batch_ksegs thousand, or what free memory (plus the scratch this
-  // context already holds) allows when a curve set first runs Barnes–Hut --
-  // kept for later calls on the same curves, so repeated certificates form
-  // the same batches and reuse the same scratch.  A curve shared by two
-  // batches is built in both.
-  int64_t budget;
-  if (prm.bh_batch_ksegs > 0) {
-    budget = (int64_t)prm.bh_batch_ksegs * 1000;
-  } else {
-    if (cv.bh_budget <= 0) {
-      size_t free_b = 0, total_b = 0;
-      FLK_HIP(hipMemGetInfo(&free_b, &total_b));
-      cv.bh_budget = (int64_t)(0.6 * (double)(free_b + ctx->bh.bytes()) / kBhBytesPerSegment);
-    }
-    budget = cv.bh_budget;
-  }
-  budget = std::max<int64_t>(budget, 1);
-  std::vector<char> in_batch(cv.ncurves, 0);
-  std::vector<int32_t> members;
-  std::vector<int64_t> batch;
-  int64_t segs = 0;
-  auto flush = [&]() {
-    if (batch.empty()) return;
-    bh_batch(ctx, pairs, batch, prm, lk, err, flags);
-    for (int32_t c : members) in_batch[c] = 0;
-    members.clear();
-    batch.clear();
-    segs = 0;
-  };
-  for (int64_t k : bh_idx) {
-    const int32_t a = pairs[2 * k], b = pairs[2 * k + 1];
-    const int64_t add = (in_batch[a] ? 0 : cv.n[a]) + (in_batch[b] || a == b ? 0 : cv.n[b]);
-    if (!batch.empty() && segs + add > budget) flush();
-    for (int32_t c : {a, b})
-      if (!in_batch[c]) {
-        in_batch[c] = 1;
-        members.push_back(c);
-        segs += cv.n[c];
-      }
-    batch.push_back(k);
-  }
-  flush();
+  for (const auto &idx : form_batches(ctx, pairs, bh_idx, prm)) bh_batch(ctx, pairs, idx, prm, lk, err, flags);
 }
 
 }  // namespace flk

@@ -563,14 +563,21 @@ DsBufs reserve_bufs(DevBuf &pairs, DevBuf &tasks, DevBuf &cost, DevBuf &theta, D
 // Uploads of the plan arrays, the tile kernel (between ev_a and ev_b when
 // given) and, with `out`, the per-pair reduction and read-back: one stream-
 // ordered sequence, launched eagerly or captured into a graph.
-void ds_enqueue(flk_ctx *ctx, const DsPlan &P, const DsPair *dp, const DsTask *tasks, const int64_t *prefix,
-                const int64_t *task0, int64_t npairs, int64_t ntasks, const DsBufs &d, hipStream_t s,
-                hipEvent_t ev_a, hipEvent_t ev_b, double *out) {
-  Curves &cv = ctx->curves;
+void ds_upload(const DsPair *dp, const DsTask *tasks, const int64_t *prefix, const int64_t *task0, int64_t npairs,
+               int64_t ntasks, const DsBufs &d, hipStream_t s) {
   FLK_HIP(hipMemcpyAsync(d.pairs, dp, sizeof(DsPair) * npairs, hipMemcpyHostToDevice, s));
   FLK_HIP(hipMemcpyAsync(d.tasks, tasks, sizeof(DsTask) * ntasks, hipMemcpyHostToDevice, s));
   FLK_HIP(hipMemcpyAsync(d.cost, prefix, sizeof(int64_t) * (ntasks + 1), hipMemcpyHostToDevice, s));
   FLK_HIP(hipMemcpyAsync(d.task0, task0, sizeof(int64_t) * (npairs + 1), hipMemcpyHostToDevice, s));
+}
+
+// With dp == nullptr the plan is already on the device (a graph uploads it
+// once, at capture).
+void ds_enqueue(flk_ctx *ctx, const DsPlan &P, const DsPair *dp, const DsTask *tasks, const int64_t *prefix,
+                const int64_t *task0, int64_t npairs, int64_t ntasks, const DsBufs &d, hipStream_t s,
+                hipEvent_t ev_a, hipEvent_t ev_b, double *out) {
+  Curves &cv = ctx->curves;
+  if (dp) ds_upload(dp, tasks, prefix, task0, npairs, ntasks, d, s);
   if (ev_a) FLK_HIP(hipEventRecord(ev_a, s));
   const float4 *hi = cv.hi.as<float4>(), *lo = cv.lo.as<float4>(), *ed = cv.ed.as<float4>();
   const dim3 grid(P.blocks), block(256);
@@ -664,11 +671,15 @@ float ds_graph_run(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, bool fp64
     const int64_t nt = (int64_t)G.tasks.size();
     const DsBufs d =
         reserve_bufs(G.d_pairs, G.d_tasks, G.d_cost, G.d_theta, G.d_corr, G.d_task0, G.d_out, npairs, nt);
+    G.h_out.reserve(sizeof(double) * npairs);
+    // the plan goes to the device once; the graph reads it there
+    ds_upload(G.dp.data(), G.tasks.data(), G.prefix.data(), G.task0.data(), npairs, nt, d, s);
+    FLK_HIP(hipStreamSynchronize(s));
     FLK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     hipGraph_t graph = nullptr;
     try {
-      ds_enqueue(ctx, P, G.dp.data(), G.tasks.data(), G.prefix.data(), G.task0.data(), npairs, nt, d, s, nullptr,
-                 nullptr, G.out.data());
+      ds_enqueue(ctx, P, nullptr, nullptr, nullptr, nullptr, npairs, nt, d, s, nullptr, nullptr,
+                 G.h_out.as<double>());
     } catch (...) {
       (void)hipStreamEndCapture(s, &graph);
       if (graph) (void)hipGraphDestroy(graph);
@@ -689,13 +700,52 @@ float ds_graph_run(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, bool fp64
   FLK_HIP(hipStreamSynchronize(s));
   float ms = 0.f;
   FLK_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-  lk_out = G.out;
+  lk_out.assign(G.h_out.as<double>(), G.h_out.as<double>() + npairs);
   *work = G.work;
   ctx->stats.ds_waves = G.waves;
   return ms;
 }
 
 }  // namespace
+
+void ds_capture_prepare(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, bool fp64, DsGraph &G) {
+  DsPlan P;
+  ds_plan(ctx, pairs, npairs, fp64, ctx->tune, 1, P);
+  G.dp = std::move(P.dp);
+  G.tasks = std::move(P.tasks);
+  G.prefix = std::move(P.prefix);
+  G.task0 = std::move(P.task0);
+  G.out.assign(npairs, 0.0);
+  G.work = P.work;
+  G.waves = P.blocks * 4;
+  G.npairs = npairs;
+  G.R = P.R;
+  G.blocks = P.blocks;
+  G.fp64 = P.fp64;
+  G.precise = P.precise;
+  G.h_out.reserve(sizeof(double) * std::max<int64_t>(npairs, 1));
+  std::fill(G.h_out.as<double>(), G.h_out.as<double>() + npairs, 0.0);
+  if (!G.tasks.empty()) {
+    const DsBufs d = reserve_bufs(G.d_pairs, G.d_tasks, G.d_cost, G.d_theta, G.d_corr, G.d_task0, G.d_out, npairs,
+                                  G.tasks.size());
+    ds_upload(G.dp.data(), G.tasks.data(), G.prefix.data(), G.task0.data(), npairs, (int64_t)G.tasks.size(), d,
+              ctx->stream);
+  }
+}
+
+void ds_capture_enqueue(flk_ctx *ctx, DsGraph &G, hipStream_t s) {
+  if (G.tasks.empty()) return;  // G.out stays zero: no segment pairs
+  DsPlan P;
+  P.R = G.R;
+  P.blocks = G.blocks;
+  P.fp64 = G.fp64;
+  P.precise = G.precise;
+  const int64_t nt = (int64_t)G.tasks.size();
+  const DsBufs d = DsBufs{G.d_pairs.as<DsPair>(), G.d_tasks.as<DsTask>(), G.d_cost.as<int64_t>(),
+                          G.d_task0.as<int64_t>(), G.d_theta.as<double>(), G.d_out.as<double>(),
+                          G.d_corr.as<int32_t>()};
+  ds_enqueue(ctx, P, nullptr, nullptr, nullptr, nullptr, G.npairs, nt, d, s, nullptr, nullptr, G.h_out.as<double>());
+}
 
 void ds_run(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, bool fp64,
             std::vector<double> &lk_out, float *ms_kernel, bool use_graph) {

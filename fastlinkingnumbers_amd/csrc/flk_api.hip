@@ -65,6 +65,7 @@ int64_t curves_layout(flk_ctx *ctx, const int64_t *offsets, int32_t ncurves) {
   cv.bh_budget = 0;
   ++ctx->curves_version;
   for (auto &g : ctx->ds_graph) g.reset();
+  ctx->cert_graph.reset();
   return nv;
 }
 
@@ -80,6 +81,7 @@ void curves_clear(flk_ctx *ctx) {
   cv.bh_budget = 0;
   ++ctx->curves_version;
   for (auto &g : ctx->ds_graph) g.reset();
+  ctx->cert_graph.reset();
 }
 
 void curves_finish(flk_ctx *ctx) {
@@ -252,6 +254,7 @@ void flk_destroy(flk_ctx *ctx) {
                          &ctx->d_boxes, &ctx->d_pls, &ctx->d_count, &ctx->d_misc,
                          &ctx->mlp.d_params, &ctx->mlp.d_norm, &ctx->mlp.d_bf16};
   for (auto *b : bufs) b->release();
+  ctx->cert_graph.reset();
   ctx->bh.release();
   ctx->stage.release();
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);

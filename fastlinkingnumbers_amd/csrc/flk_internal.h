@@ -4,6 +4,7 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <string>
 #include <memory>
@@ -27,7 +28,6 @@ struct Error {
   Error(int32_t c, std::string m) : code(c), msg(std::move(m)) {}
 };
 
-// Device allocation that grows on demand and is freed with its owner.
 // Kernel launch that optionally carries start/stop events on the dispatch
 // itself (hipExtLaunchKernelGGL); without events it is a plain launch.
 template <typename K, typename... Args>
@@ -37,13 +37,24 @@ inline void launch_timed(hipEvent_t start, hipEvent_t stop, K kernel, dim3 grid,
   else hipLaunchKernelGGL(kernel, grid, block, 0, s, args...);
 }
 
+// Allocations and frees of a tracked DevBuf (the Barnes–Hut scratch and the
+// learned criterion's weights, which a certificate graph holds pointers into)
+// bump this counter: such a graph is replayed only while it is unchanged.
+inline std::atomic<uint64_t> g_devbuf_epoch{0};
+// Set while this thread captures a graph: a DevBuf that would allocate then
+// throws instead (every buffer is reserved before the capture begins).
+inline thread_local bool t_capturing = false;
+
+// Device allocation that grows on demand and is freed with its owner.
 struct DevBuf {
   void *ptr = nullptr;
   size_t bytes = 0;
+  bool tracked = false;  // (re)allocation bumps g_devbuf_epoch
   DevBuf() = default;
+  explicit DevBuf(bool track) : tracked(track) {}
   DevBuf(const DevBuf &) = delete;
   DevBuf &operator=(const DevBuf &) = delete;
-  DevBuf(DevBuf &&o) noexcept : ptr(o.ptr), bytes(o.bytes) {
+  DevBuf(DevBuf &&o) noexcept : ptr(o.ptr), bytes(o.bytes), tracked(o.tracked) {
     o.ptr = nullptr;
     o.bytes = 0;
   }
@@ -60,14 +71,58 @@ struct DevBuf {
   ~DevBuf() { release(); }
   void reserve(size_t n) {
     if (n <= bytes) return;
+    if (t_capturing) throw Error(FLK_ERR_STATE, "device allocation during graph capture");
     release();
     if (n == 0) return;
     FLK_HIP(hipMalloc(&ptr, n));
     bytes = n;
+    if (tracked) ++g_devbuf_epoch;
   }
   template <typename T> T *as() const { return static_cast<T *>(ptr); }
   void release() {
-    if (ptr) (void)hipFree(ptr);
+    if (ptr) {
+      (void)hipFree(ptr);
+      if (tracked) ++g_devbuf_epoch;
+    }
+    ptr = nullptr;
+    bytes = 0;
+  }
+};
+
+// Pinned host allocation (grow-only): the end of a graph's read-back copies,
+// which then run as DMA without staging through pageable memory.
+struct HostBuf {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  HostBuf() = default;
+  HostBuf(const HostBuf &) = delete;
+  HostBuf &operator=(const HostBuf &) = delete;
+  HostBuf(HostBuf &&o) noexcept : ptr(o.ptr), bytes(o.bytes) {
+    o.ptr = nullptr;
+    o.bytes = 0;
+  }
+  HostBuf &operator=(HostBuf &&o) noexcept {
+    if (this != &o) {
+      release();
+      ptr = o.ptr;
+      bytes = o.bytes;
+      o.ptr = nullptr;
+      o.bytes = 0;
+    }
+    return *this;
+  }
+  ~HostBuf() { release(); }
+  void reserve(size_t n) {
+    if (n <= bytes) return;
+    if (t_capturing) throw Error(FLK_ERR_STATE, "host allocation during graph capture");
+    release();
+    if (n == 0) return;
+    FLK_HIP(hipHostMalloc(&ptr, n, 0));
+    bytes = n;
+  }
+  template <typename T> T *as() const { return static_cast<T *>(ptr); }
+  void release() {
+    if (ptr) (void)hipHostFree(ptr);
     ptr = nullptr;
     bytes = 0;
   }
@@ -139,16 +194,23 @@ struct HostStage {
 struct BhScratch {
   // forest
   DevBuf geo, mom, order, m64, range, child, flag, scan, keys0, keys1, ids0, tmp, seg, vtx, box, sp;
-  // traversal
-  DevBuf roots_a, roots_b, beta2, cnt, leaves, pred, sum, err;
+  // traversal: root items of a pass, re-run list, per-pair sums and results
+  DevBuf roots_a, roots_b, beta2, cnt, leaves, pred, sum, err, items0, redo, outv, oute, outf;
   std::vector<std::unique_ptr<DevBuf>> levels;
   unsigned long long *host_cnt = nullptr;  // pinned, 2 words per level of a traversal round trip
   int64_t dev_cap = 0;                     // items per level buffer of the device-scheduled pass
   std::vector<unsigned long long> dev_counts;  // its per-level counts, read back once per pass
+  BhScratch() {
+    each([](DevBuf &b) { b.tracked = true; });
+  }
+  DevBuf &add_level() {
+    levels.push_back(std::make_unique<DevBuf>(true));
+    return *levels.back();
+  }
   template <typename Fn> void each(Fn fn) {
     DevBuf *all[] = {&geo, &mom, &order, &m64, &range, &child, &flag, &scan, &keys0, &keys1, &ids0,
                      &tmp, &seg, &vtx, &box, &sp, &roots_a, &roots_b, &beta2, &cnt, &leaves, &pred,
-                     &sum, &err};
+                     &sum, &err, &items0, &redo, &outv, &oute, &outf};
     for (DevBuf *b : all) fn(*b);
     for (auto &b : levels) fn(*b);
   }
@@ -174,11 +236,15 @@ struct DsGraph {
   hipGraphExec_t exec = nullptr;
   int64_t work = 0;
   int32_t waves = 0;
+  int64_t npairs = 0;
+  int R = 2, blocks = 0;          // the plan's launch shape and precision
+  bool fp64 = false, precise = false;
   std::vector<DsPair> dp;
   std::vector<DsTask> tasks;
   std::vector<int64_t> prefix, task0;
   std::vector<double> out;
   DevBuf d_pairs, d_tasks, d_cost, d_theta, d_corr, d_task0, d_out;
+  HostBuf h_out;  // pinned end of the read-back (copied into out)
   void reset() {
     if (exec) (void)hipGraphExecDestroy(exec);
     exec = nullptr;
@@ -211,8 +277,11 @@ struct MlpState {
   std::vector<float> params;  // FLK_MLP_NPARAM FP32 masters
   std::vector<float> norm;    // mean[12], std[12]
   bool loaded = false;
-  DevBuf d_params, d_norm, d_bf16;
+  DevBuf d_params{true}, d_norm{true}, d_bf16;
 };
+
+// A whole Barnes–Hut or hybrid certificate captured into one graph (bh.hip).
+struct CertGraph;
 
 }  // namespace flk
 
@@ -233,6 +302,7 @@ struct flk_ctx {
   // every block resident: 880 ms against 931 ms for R = 4, 1,068 ms for R = 8)
   flk_tune tune{2, 0, 8, 0, 0, 0, {0, 0}};
   flk::DsGraph ds_graph[2];      // FP32, FP64 batches
+  std::shared_ptr<flk::CertGraph> cert_graph;  // last Barnes–Hut certificate graph
   flk::DiscResult disc;          // last flk_discretize result
   flk::BccLoad bcc;              // last flk_load_bcc result
   flk::DevBuf bcc_land;          // FP32 landing area of flk_load_bcc
@@ -257,6 +327,27 @@ float ds_time_sample(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, const f
                      int64_t *work);
 // Segment pairs of the pairs' direct-sum work.
 int64_t ds_work(flk_ctx *ctx, const int32_t *pairs, int64_t npairs);
+// A direct-sum batch recorded inside a larger graph: ds_capture_prepare plans
+// the pairs into G's host arrays, reserves G's buffers and uploads the plan
+// (call it before the capture begins); ds_capture_enqueue enqueues the tile
+// kernel, the per-pair reduction and the read-back into G.h_out, which the
+// caller copies into G.out after the launch.
+void ds_capture_prepare(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, bool fp64, DsGraph &G);
+void ds_capture_enqueue(flk_ctx *ctx, DsGraph &G, hipStream_t s);
+
+// Device-wide exclusive prefix sum (sort.hip); in and out may alias; temp
+// holds scan_temp_bytes(n).  Kernels only: capturable into a graph.
+size_t scan_temp_bytes(int64_t n);
+void exclusive_scan(const int32_t *in, int32_t *out, int64_t n, int32_t *temp, hipStream_t s);
+// Stable LSD radix sort of (key, value) pairs on bits [0, end_bit): keys and
+// vals are the input (overwritten), keys_alt and vals_alt scratch of n
+// entries, vals_out (distinct from vals and vals_alt) receives the values in
+// key order; the sorted keys are not kept.  temp holds radix_temp_bytes(n).
+// Kernels only: capturable into a graph.
+size_t radix_temp_bytes(int64_t n);
+template <typename K>
+void radix_sort_pairs(K *keys, K *keys_alt, uint32_t *vals, uint32_t *vals_alt, uint32_t *vals_out, int64_t n,
+                      int end_bit, void *temp, hipStream_t s);
 // Device address of this GPU's fixed direct-sum array (looked up once per
 // device; call with the device current).
 void *noalloc_scratch_init(int device);

@@ -1,0 +1,309 @@
+// Device-wide prefix sum and stable LSD radix sort for gfx950, written so a
+// caller can capture them into a HIP graph: no memsets, no host reads, no
+// allocation and no waits between blocks (every block finishes on its own).
+//
+// Prefix sum: three launches over tiles of 4,096 values (per-tile totals, one
+// block scanning the totals, per-tile scans with their offsets), or one block
+// for short inputs.
+//
+// Radix sort: 8-bit digits, least significant first, tiles of 4,096 keys.
+// Per digit pass: a count kernel writes every tile's digit histogram into a
+// digit-major array (digit d of tile t at d * tiles + t), the prefix sum turns
+// it into each (digit, tile) run's first output position, and a scatter
+// kernel ranks its tile's keys -- each wave walks 64 consecutive keys per step
+// and ranks equal digits with eight ballots, in key order, so the sort is
+// stable -- reorders the tile by digit in LDS and writes each digit's run
+// contiguously.  The last pass writes only the values (the caller wants the
+// permutation, not the sorted keys).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "flk_internal.h"
+
+namespace flk {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kItems = 16;
+constexpr int kTile = kBlock * kItems;      // values per block tile
+constexpr int64_t kSingleBlockMax = 65536;  // shorter inputs: one block
+
+__device__ __forceinline__ void wave_lds_order() {
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
+__device__ __forceinline__ int wave_inclusive(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int u = __shfl_up(v, d, 64);
+    if (lane >= d) v += u;
+  }
+  return v;
+}
+
+// Exclusive scan of one value per thread over the 256-thread block; *total
+// receives the block's sum.  sw: 4 ints of LDS.  Contains block barriers.
+__device__ __forceinline__ int block_exclusive(int v, int *total, int *sw) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int inc = wave_inclusive(v);
+  if (lane == 63) sw[w] = inc;
+  __syncthreads();
+  int off = 0, tot = 0;
+#pragma unroll
+  for (int k = 0; k < kBlock / 64; ++k) {
+    const int t = sw[k];
+    off += k < w ? t : 0;
+    tot += t;
+  }
+  __syncthreads();
+  *total = tot;
+  return off + inc - v;
+}
+
+// Sum of each 4,096-value tile.
+__global__ __launch_bounds__(kBlock) void scan_tile_sums(const int32_t *__restrict__ in, int64_t n,
+                                                         int32_t *__restrict__ part) {
+  __shared__ int sw[4];
+  const int64_t base = (int64_t)blockIdx.x * kTile;
+  int v = 0;
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    const int64_t g = base + i * kBlock + threadIdx.x;
+    if (g < n) v += in[g];
+  }
+  int tot = 0;
+  (void)block_exclusive(v, &tot, sw);
+  if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+// One block, exclusive scan of in[0, n) into out (may alias), chunk by chunk
+// with a running carry; each thread owns 16 consecutive values of a chunk.
+__global__ __launch_bounds__(kBlock) void scan_one_block(const int32_t *in, int32_t *out, int64_t n) {
+  __shared__ int sw[4];
+  __shared__ int32_t buf[kTile];
+  int carry = 0;
+  for (int64_t c0 = 0; c0 < n; c0 += kTile) {
+    const int m = (int)min<int64_t>(kTile, n - c0);
+    for (int i = threadIdx.x; i < m; i += kBlock) buf[i] = in[c0 + i];
+    __syncthreads();
+    int v[kItems], s = 0;
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+      const int k = threadIdx.x * kItems + i;
+      v[i] = k < m ? buf[k] : 0;
+      s += v[i];
+    }
+    int tot = 0;
+    int run = carry + block_exclusive(s, &tot, sw);
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+      const int k = threadIdx.x * kItems + i;
+      if (k < m) buf[k] = run;
+      run += v[i];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < m; i += kBlock) out[c0 + i] = buf[i];
+    __syncthreads();  // buf is refilled by the next chunk
+    carry += tot;
+  }
+}
+
+// Exclusive scan of each tile, offset by its tile's scanned total.
+__global__ __launch_bounds__(kBlock) void scan_tiles(const int32_t *in, int32_t *out, int64_t n,
+                                                     const int32_t *__restrict__ part) {
+  __shared__ int sw[4];
+  __shared__ int32_t buf[kTile];
+  const int64_t base = (int64_t)blockIdx.x * kTile;
+  const int m = (int)min<int64_t>(kTile, n - base);
+  for (int i = threadIdx.x; i < m; i += kBlock) buf[i] = in[base + i];
+  __syncthreads();
+  int v[kItems], s = 0;
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    const int k = threadIdx.x * kItems + i;
+    v[i] = k < m ? buf[k] : 0;
+    s += v[i];
+  }
+  int tot = 0;
+  int run = part[blockIdx.x] + block_exclusive(s, &tot, sw);
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    const int k = threadIdx.x * kItems + i;
+    if (k < m) buf[k] = run;
+    run += v[i];
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < m; i += kBlock) out[base + i] = buf[i];
+}
+
+// Digit histogram of one tile: per-wave LDS counters (a quarter of the
+// same-address contention), summed into counts[d * tiles + tile].
+template <typename K>
+__global__ __launch_bounds__(kBlock) void radix_count(const K *__restrict__ keys, int64_t n, int shift,
+                                                      int64_t tiles, int32_t *__restrict__ counts) {
+  __shared__ int hist[kBlock / 64][256];
+  const int w = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < 256 * (kBlock / 64); i += kBlock) (&hist[0][0])[i] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * kTile;
+#pragma unroll 4
+  for (int i = 0; i < kItems; ++i) {
+    const int64_t g = base + i * kBlock + threadIdx.x;
+    if (g < n) atomicAdd(&hist[w][(int)((keys[g] >> shift) & 255)], 1);
+  }
+  __syncthreads();
+  const int d = threadIdx.x;
+  counts[(int64_t)d * tiles + blockIdx.x] = hist[0][d] + hist[1][d] + hist[2][d] + hist[3][d];
+}
+
+// Stable scatter of one tile by the digit at `shift`.  offsets[d * tiles + t]
+// is the first output position of tile t's keys with digit d.  Wave w ranks
+// keys [w * 1024, (w + 1) * 1024) of the tile, 64 per step in key order: the
+// lanes holding the same digit find each other with eight ballots, take the
+// wave's running count of that digit, and the lowest of them advances it.
+template <typename K, bool kLast>
+__global__ __launch_bounds__(kBlock) void radix_scatter(const K *__restrict__ kin, const uint32_t *__restrict__ vin,
+                                                        int64_t n, int shift, int64_t tiles,
+                                                        const int32_t *__restrict__ offsets, K *__restrict__ kout,
+                                                        uint32_t *__restrict__ vout) {
+  constexpr int kWaves = kBlock / 64;
+  __shared__ int wcnt[kWaves][256];
+  __shared__ int dstart[256];
+  __shared__ int gbase[256];
+  __shared__ int sw[4];
+  __shared__ K skey[kTile];
+  __shared__ uint32_t sval[kTile];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < 256 * kWaves; i += kBlock) (&wcnt[0][0])[i] = 0;
+  __syncthreads();
+  const int64_t tile0 = (int64_t)blockIdx.x * kTile;
+  const int64_t wbase = tile0 + (int64_t)w * 64 * kItems;
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  K key[kItems];
+  uint32_t val[kItems];
+  int rank[kItems];
+#pragma unroll
+  for (int r = 0; r < kItems; ++r) {
+    const int64_t g = wbase + r * 64 + lane;
+    const bool ok = g < n;
+    key[r] = ok ? kin[g] : K(0);
+    val[r] = ok ? vin[g] : 0u;
+  }
+#pragma unroll
+  for (int r = 0; r < kItems; ++r) {
+    const bool ok = wbase + r * 64 + lane < n;
+    const int d = (int)((key[r] >> shift) & 255);
+    uint64_t peers = __ballot(ok);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const uint64_t m = __ballot((d >> b) & 1);
+      peers &= ((d >> b) & 1) ? m : ~m;
+    }
+    const int before = __popcll(peers & below);
+    const int seen = ok ? wcnt[w][d] : 0;
+    wave_lds_order();  // every lane has read its digit's count before one of them advances it
+    if (ok && before == 0) wcnt[w][d] = seen + __popcll(peers);
+    wave_lds_order();
+    rank[r] = ok ? seen + before : -1;
+  }
+  __syncthreads();
+  {
+    const int d = threadIdx.x;  // one digit per thread
+    int c[kWaves], t = 0;
+#pragma unroll
+    for (int k = 0; k < kWaves; ++k) {
+      c[k] = wcnt[k][d];
+      t += c[k];
+    }
+    int all = 0;
+    const int start = block_exclusive(t, &all, sw);
+    int run = start;
+#pragma unroll
+    for (int k = 0; k < kWaves; ++k) {
+      wcnt[k][d] = run;
+      run += c[k];
+    }
+    dstart[d] = start;
+    gbase[d] = offsets[(int64_t)d * tiles + blockIdx.x];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kItems; ++r)
+    if (rank[r] >= 0) {
+      const int pos = wcnt[w][(int)((key[r] >> shift) & 255)] + rank[r];
+      skey[pos] = key[r];
+      sval[pos] = val[r];
+    }
+  __syncthreads();
+  const int m = (int)min<int64_t>(kTile, n - tile0);
+  for (int i = threadIdx.x; i < m; i += kBlock) {
+    const K k = skey[i];
+    const int d = (int)((k >> shift) & 255);
+    const int64_t dst = (int64_t)gbase[d] + (i - dstart[d]);
+    if (!kLast) kout[dst] = k;
+    vout[dst] = sval[i];
+  }
+}
+
+int64_t tiles_of(int64_t n) { return (n + kTile - 1) / kTile; }
+
+}  // namespace
+
+size_t scan_temp_bytes(int64_t n) { return sizeof(int32_t) * (size_t)(tiles_of(std::max<int64_t>(n, 1)) + 16); }
+
+void exclusive_scan(const int32_t *in, int32_t *out, int64_t n, int32_t *temp, hipStream_t s) {
+  if (n <= 0) return;
+  if (n <= kSingleBlockMax) {
+    hipLaunchKernelGGL(scan_one_block, dim3(1), dim3(kBlock), 0, s, in, out, n);
+    FLK_HIP(hipGetLastError());
+    return;
+  }
+  const int64_t t = tiles_of(n);
+  hipLaunchKernelGGL(scan_tile_sums, dim3((unsigned)t), dim3(kBlock), 0, s, in, n, temp);
+  hipLaunchKernelGGL(scan_one_block, dim3(1), dim3(kBlock), 0, s, (const int32_t *)temp, temp, t);
+  hipLaunchKernelGGL(scan_tiles, dim3((unsigned)t), dim3(kBlock), 0, s, in, out, n, (const int32_t *)temp);
+  FLK_HIP(hipGetLastError());
+}
+
+size_t radix_temp_bytes(int64_t n) {
+  const int64_t cells = 256 * tiles_of(std::max<int64_t>(n, 1));
+  return sizeof(int32_t) * (size_t)cells + scan_temp_bytes(cells);
+}
+
+template <typename K>
+void radix_sort_pairs(K *keys, K *keys_alt, uint32_t *vals, uint32_t *vals_alt, uint32_t *vals_out, int64_t n,
+                      int end_bit, void *temp, hipStream_t s) {
+  if (n <= 0) return;
+  if (n >= (int64_t)INT32_MAX) throw Error(FLK_ERR_ARG, "radix sort: too many keys");
+  const int64_t tiles = tiles_of(n);
+  const int64_t cells = 256 * tiles;
+  int32_t *counts = static_cast<int32_t *>(temp);
+  int32_t *part = counts + cells;
+  const int passes = std::max(1, (end_bit + 7) / 8);
+  K *kbuf[2] = {keys, keys_alt};
+  uint32_t *vbuf[2] = {vals, vals_alt};
+  for (int p = 0; p < passes; ++p) {
+    const K *ki = kbuf[p & 1];
+    const uint32_t *vi = vbuf[p & 1];
+    hipLaunchKernelGGL(radix_count<K>, dim3((unsigned)tiles), dim3(kBlock), 0, s, ki, n, 8 * p, tiles, counts);
+    FLK_HIP(hipGetLastError());
+    exclusive_scan(counts, counts, cells, part, s);
+    if (p + 1 == passes)
+      hipLaunchKernelGGL((radix_scatter<K, true>), dim3((unsigned)tiles), dim3(kBlock), 0, s, ki, vi, n, 8 * p,
+                         tiles, (const int32_t *)counts, (K *)nullptr, vals_out);
+    else
+      hipLaunchKernelGGL((radix_scatter<K, false>), dim3((unsigned)tiles), dim3(kBlock), 0, s, ki, vi, n, 8 * p,
+                         tiles, (const int32_t *)counts, kbuf[(p + 1) & 1], vbuf[(p + 1) & 1]);
+    FLK_HIP(hipGetLastError());
+  }
+}
+
+template void radix_sort_pairs<uint32_t>(uint32_t *, uint32_t *, uint32_t *, uint32_t *, uint32_t *, int64_t, int,
+                                         void *, hipStream_t);
+template void radix_sort_pairs<uint64_t>(uint64_t *, uint64_t *, uint32_t *, uint32_t *, uint32_t *, int64_t, int,
+                                         void *, hipStream_t);
+
+}  // namespace flk

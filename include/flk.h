@@ -63,8 +63,10 @@ typedef struct flk_params {
   int32_t timing;       /* 1: time every Barnes–Hut level launch with HIP events
                            (ms_bh_level, ms_mlp); the per-launch
                            events cost ~15% of a traversal, so 0 by default */
-  int32_t graph;        /* 1 (default): a direct-sum batch is captured in a HIP
-                           graph and replayed when the same pairs come again;
+  int32_t graph;        /* 1 (default): a direct-sum batch, or a whole Barnes–Hut
+                           certificate (its direct-sum part, tree builds and
+                           traversal passes), is captured in a HIP graph and
+                           replayed when the same pairs come again;
                            0: launched eagerly every time */
   int32_t reserved[5];
 } flk_params;
@@ -92,7 +94,12 @@ typedef struct flk_stats {
   int64_t bh_host_passes;   /* Barnes–Hut passes that ran level by level from the host
                                (per-launch timing, or the device-scheduled pass's
                                buffers overflowed) instead of all levels enqueued at once */
-  int64_t reserved[4];
+  int64_t graph_launches;   /* Barnes–Hut certificates run as one captured HIP graph (params.graph:
+                               the direct-sum part, every tree build and both traversal passes);
+                               their device time is ms_kernel, and ms_bh_build/ms_bh_traverse
+                               stay 0 */
+  double ms_graph_launch;   /* host time of that hipGraphLaunch call */
+  int64_t reserved[2];
 } flk_stats;
 
 /* Fills *p with the reference defaults (method Barnes–Hut, beta 2/10, ...). */
