@@ -69,39 +69,37 @@ __device__ __forceinline__ uint64_t spread16(uint64_t v) {
   return v;
 }
 
-// One thread per segment of the Barnes–Hut curves: key = curve rank <<
-// (3 * bits) | Morton code of the midpoint in the curve's box, `bits` per
-// axis.  cbox holds per curve the box corner (3 doubles) and cells / extent
-// per axis (0 for a flat axis).  Segments sharing a cell keep their order
-// along the curve (the sort is stable).
-template <typename K>
-__global__ __launch_bounds__(256) void bh_keys(const double4 *__restrict__ pts,
-                                               const int64_t *__restrict__ seg_off,
-                                               const int64_t *__restrict__ vtx_off,
-                                               const double *__restrict__ cbox, int32_t ncurves, int bits,
-                                               int64_t nseg, K *__restrict__ keys,
+// Sort keys of the Barnes–Hut segments, one block per tile of the segmented
+// sort (a tile lies within one curve, tcurve[b]): the Morton code of the
+// segment midpoint in the curve's box, 10 bits per axis; the value is the
+// segment's first vertex.  cbox holds per curve the box corner (3 doubles)
+// and 1024 / extent per axis (0 for a flat axis).  The segmented sort keeps
+// every curve's segments in its own range, and segments sharing a cell keep
+// their order along the curve (the sort is stable).
+__global__ __launch_bounds__(256) void bh_keys(const double4 *__restrict__ pts, const int64_t *__restrict__ seg_off,
+                                               const int64_t *__restrict__ vtx_off, const double *__restrict__ cbox,
+                                               const RadixTile *__restrict__ tiles,
+                                               const int32_t *__restrict__ tcurve, uint32_t *__restrict__ keys,
                                                uint32_t *__restrict__ vals) {
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= nseg) return;
-  int lo = 0, hi = ncurves;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (seg_off[mid] <= g) lo = mid;
-    else hi = mid;
+  const RadixTile tl = tiles[blockIdx.x];
+  const int c = tcurve[blockIdx.x];
+  const int64_t v0 = vtx_off[c] - seg_off[c];
+  const double lo[3] = {cbox[6 * c], cbox[6 * c + 1], cbox[6 * c + 2]};
+  const double sc[3] = {cbox[6 * c + 3], cbox[6 * c + 4], cbox[6 * c + 5]};
+  for (int o = threadIdx.x; o < tl.count; o += 256) {
+    const int64_t g = (int64_t)tl.start + o;
+    const int64_t v = v0 + g;
+    const double4 p = pts[v], q = pts[v + 1];
+    const double m[3] = {0.5 * (p.x + q.x), 0.5 * (p.y + q.y), 0.5 * (p.z + q.z)};
+    uint32_t code = 0;
+    for (int d = 0; d < 3; ++d) {
+      const double u = (m[d] - lo[d]) * sc[d];
+      const uint64_t qd = (uint64_t)fmin(fmax(u, 0.0), 1023.0);
+      code |= (uint32_t)(spread16(qd) << (2 - d));
+    }
+    keys[g] = code;
+    vals[g] = (uint32_t)v;
   }
-  const int c = lo;
-  const int64_t v = vtx_off[c] + (g - seg_off[c]);
-  const double4 p = pts[v], q = pts[v + 1];
-  const double m[3] = {0.5 * (p.x + q.x), 0.5 * (p.y + q.y), 0.5 * (p.z + q.z)};
-  const double top = (double)((1 << bits) - 1);
-  uint64_t code = 0;
-  for (int d = 0; d < 3; ++d) {
-    const double u = (m[d] - cbox[6 * c + d]) * cbox[6 * c + 3 + d];
-    const uint64_t qd = (uint64_t)fmin(fmax(u, 0.0), top);
-    code |= spread16(qd) << (2 - d);
-  }
-  keys[g] = (K)(((uint64_t)c << (3 * bits)) | code);
-  vals[g] = (uint32_t)v;
 }
 
 // Splits every node of one level with more than `leaf` segments: children get
@@ -136,29 +134,37 @@ __global__ __launch_bounds__(256) void bh_split(const BhRange *__restrict__ lev,
 __host__ __device__ __forceinline__ int qidx(int i, int j) { return sym_idx(i, j); }
 
 // Endpoints of every segment in sorted order, gathered once after the sort
-// (six FP64 values per segment, copied unchanged from the vertex array), so
-// the moment kernels read each node's segments as one contiguous run.
+// (six values per segment, copied unchanged from the vertex array), so the
+// moment kernels read each node's segments as one contiguous run.  SP is
+// double2 (48 bytes per segment), or float2 (24 bytes) when every stored
+// coordinate is exact in FP32 (the curve set has no FP32 lo parts): the FP64
+// values read back are then the same bits, at half the traffic.
+__device__ __forceinline__ double2 mk2(double2, double x, double y) { return make_double2(x, y); }
+__device__ __forceinline__ float2 mk2(float2, double x, double y) { return make_float2((float)x, (float)y); }
+
+template <typename SP>
 __global__ __launch_bounds__(256) void bh_stage_segments(const uint32_t *__restrict__ order,
                                                          const double4 *__restrict__ pts, int64_t nseg,
-                                                         double2 *__restrict__ sp) {
+                                                         SP *__restrict__ sp) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nseg) return;
   const uint32_t v = order[s];
   const double4 a = pts[v], b = pts[v + 1];
-  sp[3 * s] = make_double2(a.x, a.y);
-  sp[3 * s + 1] = make_double2(a.z, b.x);
-  sp[3 * s + 2] = make_double2(b.y, b.z);
+  sp[3 * s] = mk2(SP{}, a.x, a.y);
+  sp[3 * s + 1] = mk2(SP{}, a.z, b.x);
+  sp[3 * s + 2] = mk2(SP{}, b.y, b.z);
 }
 
-__device__ __forceinline__ void seg_ends(const double2 *__restrict__ sp, int s, double (&pa)[3],
-                                         double (&pb)[3]) {
-  const double2 u = sp[3 * (int64_t)s], v = sp[3 * (int64_t)s + 1], w = sp[3 * (int64_t)s + 2];
+template <typename SP>
+__device__ __forceinline__ void seg_ends(const SP *__restrict__ sp, int s, double (&pa)[3], double (&pb)[3]) {
+  const SP u = sp[3 * (int64_t)s], v = sp[3 * (int64_t)s + 1], w = sp[3 * (int64_t)s + 2];
   pa[0] = u.x; pa[1] = u.y; pa[2] = v.x;
   pb[0] = v.y; pb[1] = w.x; pb[2] = w.y;
 }
 
 // Bounding box of the segments [r.lo, r.hi) of a leaf bucket.
-__device__ void leaf_box(BhRange r, const double2 *__restrict__ sp, double (&box)[6]) {
+template <typename SP>
+__device__ void leaf_box(BhRange r, const SP *__restrict__ sp, double (&box)[6]) {
   for (int t = 0; t < 3; ++t) {
     box[t] = DBL_MAX;
     box[3 + t] = -DBL_MAX;
@@ -216,7 +222,8 @@ __device__ __forceinline__ void node_records(const Mom64 &M, int2 ch, BhRange r,
 // (at most `direct` terms, the precision of the FP32 records they end in).
 // Both moment kernels use it, so a node's records depend only on its own
 // segments, never on which launch computed them.
-__device__ __forceinline__ void direct_records(BhRange rr, int2 ch, const double2 *__restrict__ sp, BhGeo &gr,
+template <typename SP>
+__device__ __forceinline__ void direct_records(BhRange rr, int2 ch, const SP *__restrict__ sp, BhGeo &gr,
                                                BhMom &m) {
   double box[6];
   leaf_box(rr, sp, box);
@@ -342,9 +349,10 @@ __device__ __forceinline__ void wave_store_m64(const Mom64 &M, unsigned long lon
 // A wave's 64 consecutive records leave through a 4 KB LDS buffer, so every
 // store instruction writes 1 KB of contiguous memory instead of one 16-byte
 // piece of each lane's own record.
+template <typename SP>
 __global__ __launch_bounds__(256) void bh_moments_direct(const BhRange *__restrict__ range, int32_t base,
                                                          int32_t n, const int2 *__restrict__ child,
-                                                         const double2 *__restrict__ sp, BhGeo *__restrict__ geo,
+                                                         const SP *__restrict__ sp, BhGeo *__restrict__ geo,
                                                          BhMom *__restrict__ mom) {
   __shared__ float4 stage[4][256];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -358,8 +366,9 @@ __global__ __launch_bounds__(256) void bh_moments_direct(const BhRange *__restri
 }
 
 // FP64 moments of a node above `direct` segments from its two children.
+template <typename SP>
 __device__ void node_moments(BhRange rr, int2 ch, int32_t direct, const BhRange *__restrict__ range,
-                             const double2 *__restrict__ sp, const Mom64 *__restrict__ m64,
+                             const SP *__restrict__ sp, const Mom64 *__restrict__ m64,
                              const BhMom *__restrict__ mom, Mom64 &M) {
   for (int t = 0; t < 3; ++t) M.c[t] = 0;
   for (int t = 0; t < 9; ++t) M.D[t] = 0;
@@ -413,10 +422,11 @@ __device__ void node_moments(BhRange rr, int2 ch, int32_t direct, const BhRange 
 // Eqs. 21-23): a larger child's FP64 record, or a small child's FP32 record.  Only nodes above
 // `direct` segments write FP64 records, so the bottom levels never round-trip
 // them through memory.  Children are read one at a time.
+template <typename SP>
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) void bh_moments(const BhRange *__restrict__ range, int32_t n,
                                                   int32_t base, int32_t direct,
                                                   const int2 *__restrict__ child,
-                                                  const double2 *__restrict__ sp,
+                                                  const SP *__restrict__ sp,
                                                   Mom64 *__restrict__ m64, BhGeo *__restrict__ geo,
                                                   BhMom *__restrict__ mom) {
   __shared__ float4 stage[2][18 * 16];
@@ -1175,15 +1185,15 @@ struct ForestPlan {
   std::vector<int64_t> seg_off, vtx_off;
   std::vector<double> cbox;
   std::vector<BhRange> roots;
+  std::vector<RadixTile> tiles;  // the segmented sort's tiles (one curve each)
+  std::vector<int32_t> tcurve;   // each tile's curve rank
   int64_t S = 0;
-  int end_bit = 0;
-  bool key32 = true;
   std::vector<int32_t> level_base, level_n;
   std::vector<int64_t> level_split, level_max;
   int32_t total = 0;
   int lev_top = 0;
-  // In a graph: seg_off, vtx_off, cbox and roots uploaded once (dsrc, in that
-  // order), copied device to device on every replay.
+  // In a graph: seg_off, vtx_off, cbox, roots, tiles and tcurve uploaded once
+  // (dsrc, in that order), copied device to device on every replay.
   DevBuf dconst;
   std::vector<const void *> dsrc;
 };
@@ -1230,15 +1240,11 @@ void plan_forest(flk_ctx *ctx, const std::vector<int32_t> &curves_in, int leaf, 
     P.seg_off[k + 1] = P.seg_off[k] + cv.n[P.cs[k]];
     P.vtx_off[k] = cv.off[P.cs[k]];
   }
-  // Sort keys: curve rank and a Morton code of 10 bits per axis relative to
-  // the curve's box (a curve's tree never depends on the batch around it);
-  // 4-byte keys when the rank fits the two bits left (at most 4 curves).
-  int curve_bits = 0;
-  while ((1LL << curve_bits) < nc) ++curve_bits;
-  const int axis_bits = 10;
-  P.end_bit = 3 * axis_bits + curve_bits;
-  P.key32 = P.end_bit <= 32;
-  const double cells = (double)(1 << axis_bits);
+  // Sort keys: a Morton code of 10 bits per axis relative to the curve's box
+  // (a curve's tree never depends on the batch around it), sorted within
+  // each curve's range by the segmented sort: 30-bit keys, four passes,
+  // whatever the number of curves.
+  const double cells = 1024.0;
   for (int32_t k = 0; k < nc; ++k)
     for (int d = 0; d < 3; ++d) {
       const double lo = (cv.box[6 * P.cs[k] + d] - cv.origin[d]) * cv.scale;
@@ -1250,6 +1256,10 @@ void plan_forest(flk_ctx *ctx, const std::vector<int32_t> &curves_in, int leaf, 
   if (S >= INT32_MAX / 2) throw Error(FLK_ERR_ARG, "too many Barnes-Hut segments");
   P.roots.resize(nc);
   for (int32_t k = 0; k < nc; ++k) P.roots[k] = BhRange{(int32_t)P.seg_off[k], (int32_t)P.seg_off[k + 1]};
+  const int64_t ntiles = segmented_tiles(P.seg_off.data(), nc, P.tiles);
+  P.tcurve.resize(ntiles);
+  for (int32_t k = 0, t = 0; k < nc; ++k)
+    for (; t < ntiles && P.tiles[t].start < P.seg_off[k + 1]; ++t) P.tcurve[t] = k;
   // Splits per level from the segment counts alone: a range of m > leaf
   // segments splits into floor(m/2) and m - floor(m/2), so every level of a
   // curve's tree holds at most two distinct range sizes (m and m + 1).  With
@@ -1308,13 +1318,15 @@ void plan_forest(flk_ctx *ctx, const std::vector<int32_t> &curves_in, int leaf, 
   B.seg.reserve(sizeof(int64_t) * (nc + 1));
   B.vtx.reserve(sizeof(int64_t) * nc);
   B.box.reserve(sizeof(double) * 6 * nc);
-  B.keys0.reserve((P.key32 ? 4 : 8) * S);
-  B.keys1.reserve((P.key32 ? 4 : 8) * S);
+  B.keys0.reserve(sizeof(uint32_t) * S);
+  B.keys1.reserve(sizeof(uint32_t) * S);
+  B.rtiles.reserve(sizeof(RadixTile) * ntiles);
+  B.rcurve.reserve(sizeof(int32_t) * ntiles);
   B.ids0.reserve(sizeof(uint32_t) * S);
   B.order.reserve(sizeof(uint32_t) * S);
   B.flag.reserve(sizeof(int32_t) * std::max<int64_t>(S + nc + 1, widest));
   B.scan.reserve(sizeof(int32_t) * std::max<int64_t>(S + nc + 1, widest));  // also the sort's value scratch
-  B.tmp.reserve(std::max(radix_temp_bytes(S), scan_temp_bytes(widest)));
+  B.tmp.reserve(std::max(radix_temp_bytes(S, ntiles), scan_temp_bytes(widest)));
   B.sp.reserve(sizeof(double2) * 3 * S);
   B.range.reserve(sizeof(BhRange) * (2 * S + nc));
   B.child.reserve(sizeof(int2) * (2 * S + nc));
@@ -1323,28 +1335,16 @@ void plan_forest(flk_ctx *ctx, const std::vector<int32_t> &curves_in, int leaf, 
   B.mom.reserve(sizeof(BhMom) * total);
 }
 
-void enqueue_forest(flk_ctx *ctx, const ForestPlan &P, hipStream_t s) {
-  const int32_t nc = (int32_t)P.cs.size();
-  if (nc == 0) return;
+// The end-point gather, level-by-level node creation and the moments of a
+// planned forest, with the segments' end points staged as SP.
+template <typename SP>
+void enqueue_moments(flk_ctx *ctx, const ForestPlan &P, SP *sp, hipStream_t s) {
   Curves &cv = ctx->curves;
   BhScratch &B = ctx->bh;
   const int64_t S = P.S;
-  put(B.seg.ptr, P.seg_off.data(), P.dsrc, 0, sizeof(int64_t) * (nc + 1), s);
-  put(B.vtx.ptr, P.vtx_off.data(), P.dsrc, 1, sizeof(int64_t) * nc, s);
-  put(B.box.ptr, P.cbox.data(), P.dsrc, 2, sizeof(double) * 6 * nc, s);
-  auto sort_keys = [&](auto *k0, auto *k1) {
-    using K = std::remove_pointer_t<decltype(k0)>;
-    hipLaunchKernelGGL(bh_keys<K>, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, s, cv.dpts.as<double4>(),
-                       B.seg.as<int64_t>(), B.vtx.as<int64_t>(), B.box.as<double>(), nc, 10, S, k0,
-                       B.ids0.as<uint32_t>());
-    FLK_HIP(hipGetLastError());
-    radix_sort_pairs<K>(k0, k1, B.ids0.as<uint32_t>(), B.scan.as<uint32_t>(), B.order.as<uint32_t>(), S, P.end_bit,
-                        B.tmp.ptr, s);
-  };
-  if (P.key32) sort_keys(B.keys0.as<uint32_t>(), B.keys1.as<uint32_t>());
-  else sort_keys(B.keys0.as<uint64_t>(), B.keys1.as<uint64_t>());
-  hipLaunchKernelGGL(bh_stage_segments, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, s, B.order.as<uint32_t>(),
-                     cv.dpts.as<double4>(), S, B.sp.as<double2>());
+  const int32_t nc = (int32_t)P.cs.size();
+  hipLaunchKernelGGL(bh_stage_segments<SP>, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, s,
+                     B.order.as<uint32_t>(), cv.dpts.as<double4>(), S, sp);
   FLK_HIP(hipGetLastError());
   // level-by-level node creation
   put(B.range.ptr, P.roots.data(), P.dsrc, 3, sizeof(BhRange) * nc, s);
@@ -1363,18 +1363,41 @@ void enqueue_forest(flk_ctx *ctx, const ForestPlan &P, hipStream_t s) {
   }
   if (P.lev_top < (int)P.level_base.size()) {
     const int32_t b0 = P.level_base[P.lev_top], nd = P.total - b0;
-    hipLaunchKernelGGL(bh_moments_direct, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, s,
-                       B.range.as<BhRange>(), b0, nd, B.child.as<int2>(), B.sp.as<double2>(), B.geo.as<BhGeo>(),
+    hipLaunchKernelGGL(bh_moments_direct<SP>, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, s,
+                       B.range.as<BhRange>(), b0, nd, B.child.as<int2>(), (const SP *)sp, B.geo.as<BhGeo>(),
                        B.mom.as<BhMom>());
     FLK_HIP(hipGetLastError());
   }
   for (int lev = P.lev_top - 1; lev >= 0; --lev) {
     const int32_t n = P.level_n[lev];
-    hipLaunchKernelGGL(bh_moments, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, B.range.as<BhRange>(), n,
-                       P.level_base[lev], P.direct, B.child.as<int2>(), B.sp.as<double2>(), B.m64.as<Mom64>(),
+    hipLaunchKernelGGL(bh_moments<SP>, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, B.range.as<BhRange>(),
+                       n, P.level_base[lev], P.direct, B.child.as<int2>(), (const SP *)sp, B.m64.as<Mom64>(),
                        B.geo.as<BhGeo>(), B.mom.as<BhMom>());
     FLK_HIP(hipGetLastError());
   }
+}
+
+void enqueue_forest(flk_ctx *ctx, const ForestPlan &P, hipStream_t s) {
+  const int32_t nc = (int32_t)P.cs.size();
+  if (nc == 0) return;
+  Curves &cv = ctx->curves;
+  BhScratch &B = ctx->bh;
+  const int64_t S = P.S;
+  put(B.seg.ptr, P.seg_off.data(), P.dsrc, 0, sizeof(int64_t) * (nc + 1), s);
+  put(B.vtx.ptr, P.vtx_off.data(), P.dsrc, 1, sizeof(int64_t) * nc, s);
+  put(B.box.ptr, P.cbox.data(), P.dsrc, 2, sizeof(double) * 6 * nc, s);
+  const int64_t ntiles = (int64_t)P.tiles.size();
+  put(B.rtiles.ptr, P.tiles.data(), P.dsrc, 4, sizeof(RadixTile) * ntiles, s);
+  put(B.rcurve.ptr, P.tcurve.data(), P.dsrc, 5, sizeof(int32_t) * ntiles, s);
+  hipLaunchKernelGGL(bh_keys, dim3((unsigned)ntiles), dim3(256), 0, s, cv.dpts.as<double4>(), B.seg.as<int64_t>(),
+                     B.vtx.as<int64_t>(), B.box.as<double>(), B.rtiles.as<RadixTile>(), B.rcurve.as<int32_t>(),
+                     B.keys0.as<uint32_t>(), B.ids0.as<uint32_t>());
+  FLK_HIP(hipGetLastError());
+  radix_sort_pairs<uint32_t>(B.keys0.as<uint32_t>(), B.keys1.as<uint32_t>(), B.ids0.as<uint32_t>(),
+                             B.scan.as<uint32_t>(), B.order.as<uint32_t>(), S, 30, B.tmp.ptr, s,
+                             B.rtiles.as<RadixTile>(), ntiles);
+  if (cv.inexact) enqueue_moments(ctx, P, B.sp.as<double2>(), s);
+  else enqueue_moments(ctx, P, B.sp.as<float2>(), s);
 }
 
 void forest_from_plan(flk_ctx *ctx, const ForestPlan &P, Forest &F) {
@@ -1838,7 +1861,9 @@ struct BatchPlan {
                               {{forest.seg_off.data(), 8 * forest.seg_off.size()},
                                {forest.vtx_off.data(), 8 * forest.vtx_off.size()},
                                {forest.cbox.data(), 8 * forest.cbox.size()},
-                               {forest.roots.data(), sizeof(BhRange) * forest.roots.size()}},
+                               {forest.roots.data(), sizeof(BhRange) * forest.roots.size()},
+                               {forest.tiles.data(), sizeof(RadixTile) * forest.tiles.size()},
+                               {forest.tcurve.data(), 4 * forest.tcurve.size()}},
                               s);
   }
   void collect() {  // pinned read-back into the vectors

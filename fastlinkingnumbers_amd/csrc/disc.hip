@@ -352,18 +352,29 @@ __global__ __launch_bounds__(256) void disc_split(const double *__restrict__ q, 
   }
 }
 
-// Sort key of a retired piece: global segment index, then t_lo as the
-// integer t_lo * 2^R (t_lo is a dyadic fraction with at most R bits).
+// Sort keys of a retired piece, for two stable passes: t_lo as the integer
+// t_lo * 2^R (t_lo is a dyadic fraction with at most R <= 55 bits), then the
+// global segment index.  Two passes keep the order (segment, t_lo) for any
+// segment count and round count the reference allows.
 __global__ __launch_bounds__(256) void disc_keys(const DCurve *__restrict__ cs, const int32_t *__restrict__ rc,
                                                  const int32_t *__restrict__ rs, const double *__restrict__ rlo,
-                                                 int64_t n, int R, uint64_t *__restrict__ keys,
-                                                 int32_t *__restrict__ ids) {
+                                                 int64_t n, int R, uint64_t *__restrict__ tkey,
+                                                 uint32_t *__restrict__ ids) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const uint64_t g = (uint64_t)(cs[rc[i]].seg0 + rs[i]);
-  const uint64_t t = (uint64_t)ldexp(rlo[i], R);
-  keys[i] = (g << R) | t;
-  ids[i] = (int32_t)i;
+  tkey[i] = (uint64_t)ldexp(rlo[i], R);
+  ids[i] = (uint32_t)i;
+}
+
+// Second pass's keys in the first pass's order: the global segment of piece
+// perm[k].
+__global__ __launch_bounds__(256) void disc_seg_keys(const DCurve *__restrict__ cs, const int32_t *__restrict__ rc,
+                                                     const int32_t *__restrict__ rs, const uint32_t *__restrict__ perm,
+                                                     int64_t n, uint32_t *__restrict__ gkey) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t i = perm[k];
+  gkey[k] = (uint32_t)(cs[rc[i]].seg0 + rs[i]);
 }
 
 // Output vertex of sorted retired piece k: its start point at out[k + the
@@ -606,22 +617,25 @@ void discretize_run(flk_ctx *ctx, const double *ctrl, const int64_t *offsets, co
   // sort the retired pieces by (global segment, t_lo) and evaluate the vertices
   const int R = res.rounds;
   int gbits = 1;
-  while (gbits < 63 && ((int64_t)1 << gbits) < nseg) ++gbits;
-  if (gbits + R > 64) throw Error(FLK_ERR_ARG, "too many discretization rounds for the sort key");
-  DevBuf d_k0, d_k1, d_i0, d_i1, d_ob, d_open, d_out;
+  while (gbits < 32 && ((int64_t)1 << gbits) < nseg) ++gbits;
+  DevBuf d_k0, d_k1, d_i0, d_i1, d_i2, d_ob, d_open, d_out;
   d_k0.reserve(sizeof(uint64_t) * nret);
   d_k1.reserve(sizeof(uint64_t) * nret);
-  d_i0.reserve(sizeof(int32_t) * nret);
-  d_i1.reserve(sizeof(int32_t) * nret);
+  d_i0.reserve(sizeof(uint32_t) * nret);
+  d_i1.reserve(sizeof(uint32_t) * nret);
+  d_i2.reserve(sizeof(uint32_t) * nret);
+  d_tmp.reserve(std::max<size_t>(radix_temp_bytes(nret), 16));
   hipLaunchKernelGGL(disc_keys, dim3(blocks_for(nret)), dim3(256), 0, s, dcs, ret.c.as<int32_t>(),
-                     ret.s.as<int32_t>(), ret.lo.as<double>(), nret, R, d_k0.as<uint64_t>(), d_i0.as<int32_t>());
+                     ret.s.as<int32_t>(), ret.lo.as<double>(), nret, R, d_k0.as<uint64_t>(), d_i0.as<uint32_t>());
   FLK_HIP(hipGetLastError());
-  size_t tb = 0;
-  FLK_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, d_k0.as<uint64_t>(), d_k1.as<uint64_t>(),
-                                             d_i0.as<int32_t>(), d_i1.as<int32_t>(), (int)nret, 0, gbits + R, s));
-  d_tmp.reserve(std::max<size_t>(tb, 16));
-  FLK_HIP(hipcub::DeviceRadixSort::SortPairs(d_tmp.ptr, tb, d_k0.as<uint64_t>(), d_k1.as<uint64_t>(),
-                                             d_i0.as<int32_t>(), d_i1.as<int32_t>(), (int)nret, 0, gbits + R, s));
+  // stable by t_lo (R bits), then stable by segment: d_i1 = the order
+  radix_sort_pairs<uint64_t>(d_k0.as<uint64_t>(), d_k1.as<uint64_t>(), d_i0.as<uint32_t>(), d_i1.as<uint32_t>(),
+                             d_i2.as<uint32_t>(), nret, std::max(R, 1), d_tmp.ptr, s);
+  hipLaunchKernelGGL(disc_seg_keys, dim3(blocks_for(nret)), dim3(256), 0, s, dcs, ret.c.as<int32_t>(),
+                     ret.s.as<int32_t>(), d_i2.as<uint32_t>(), nret, d_k0.as<uint32_t>());
+  FLK_HIP(hipGetLastError());
+  radix_sort_pairs<uint32_t>(d_k0.as<uint32_t>(), d_k1.as<uint32_t>(), d_i2.as<uint32_t>(), d_i0.as<uint32_t>(),
+                             d_i1.as<uint32_t>(), nret, gbits, d_tmp.ptr, s);
   // per-curve retired counts from the piece ranges: pieces of curve c are the
   // sorted positions with keys in [seg0(c), seg0(c) + nseg(c)) << R
   std::vector<int32_t> per_curve(nc, 0);

@@ -193,7 +193,7 @@ struct HostStage {
 // FP64 build records and the traversal queues.
 struct BhScratch {
   // forest
-  DevBuf geo, mom, order, m64, range, child, flag, scan, keys0, keys1, ids0, tmp, seg, vtx, box, sp;
+  DevBuf geo, mom, order, m64, range, child, flag, scan, keys0, keys1, ids0, tmp, seg, vtx, box, sp, rtiles, rcurve;
   // traversal: root items of a pass, re-run list, per-pair sums and results
   DevBuf roots_a, roots_b, beta2, cnt, leaves, pred, sum, err, items0, redo, outv, oute, outf;
   std::vector<std::unique_ptr<DevBuf>> levels;
@@ -209,7 +209,7 @@ struct BhScratch {
   }
   template <typename Fn> void each(Fn fn) {
     DevBuf *all[] = {&geo, &mom, &order, &m64, &range, &child, &flag, &scan, &keys0, &keys1, &ids0,
-                     &tmp, &seg, &vtx, &box, &sp, &roots_a, &roots_b, &beta2, &cnt, &leaves, &pred,
+                     &tmp, &seg, &vtx, &box, &sp, &rtiles, &rcurve, &roots_a, &roots_b, &beta2, &cnt, &leaves, &pred,
                      &sum, &err, &items0, &redo, &outv, &oute, &outf};
     for (DevBuf *b : all) fn(*b);
     for (auto &b : levels) fn(*b);
@@ -342,12 +342,22 @@ void exclusive_scan(const int32_t *in, int32_t *out, int64_t n, int32_t *temp, h
 // Stable LSD radix sort of (key, value) pairs on bits [0, end_bit): keys and
 // vals are the input (overwritten), keys_alt and vals_alt scratch of n
 // entries, vals_out (distinct from vals and vals_alt) receives the values in
-// key order; the sorted keys are not kept.  temp holds radix_temp_bytes(n).
-// Kernels only: capturable into a graph.
-size_t radix_temp_bytes(int64_t n);
+// key order; the sorted keys are not kept.  temp holds
+// radix_temp_bytes(n, table_tiles).  With a device tile table (from
+// segmented_tiles) the sort is segmented: every key stays within its segment
+// of the input, sorted there.  Kernels only: capturable into a graph.
+struct RadixTile {
+  int32_t start, count;  // keys [start, start + count), count <= 4096
+  int32_t cbase, stride; // digit d's count at cbase + d * stride
+};
+size_t radix_temp_bytes(int64_t n, int64_t table_tiles = 0);
+// Tiles of the segments [seg_off[c], seg_off[c+1]) (host table); returns
+// their number.
+int64_t segmented_tiles(const int64_t *seg_off, int64_t nseg, std::vector<RadixTile> &table);
 template <typename K>
 void radix_sort_pairs(K *keys, K *keys_alt, uint32_t *vals, uint32_t *vals_alt, uint32_t *vals_out, int64_t n,
-                      int end_bit, void *temp, hipStream_t s);
+                      int end_bit, void *temp, hipStream_t s, const RadixTile *table = nullptr,
+                      int64_t table_tiles = 0);
 // Device address of this GPU's fixed direct-sum array (looked up once per
 // device; call with the device current).
 void *noalloc_scratch_init(int device);

@@ -6,7 +6,8 @@
 // block scanning the totals, per-tile scans with their offsets), or one block
 // for short inputs.
 //
-// Radix sort: 8-bit digits, least significant first, tiles of 4,096 keys.
+// Radix sort: 8-bit digits, least significant first, tiles of 4,096 keys
+// (or, segmented, tiles that never straddle a segment boundary).
 // Per digit pass: a count kernel writes every tile's digit histogram into a
 // digit-major array (digit d of tile t at d * tiles + t), the prefix sum turns
 // it into each (digit, tile) run's first output position, and a scatter
@@ -139,36 +140,49 @@ __global__ __launch_bounds__(kBlock) void scan_tiles(const int32_t *in, int32_t 
   for (int i = threadIdx.x; i < m; i += kBlock) out[base + i] = buf[i];
 }
 
+// A block's tile: its first key, key count, and where its digit counts go
+// (digit d at cbase + d * stride).  Without a tile table the tiles are the
+// consecutive 4,096-key pieces of the input and the counts digit-major over
+// all tiles; a segmented sort passes a table whose tiles never straddle a
+// segment, with each segment's counts in a block of their own (digit-major
+// within it), so the prefix sum keeps every key inside its segment.
+__device__ __forceinline__ RadixTile tile_of(const RadixTile *__restrict__ table, int64_t n, int64_t tiles) {
+  if (table) return table[blockIdx.x];
+  const int64_t start = (int64_t)blockIdx.x * kTile;
+  return RadixTile{(int32_t)start, (int32_t)min<int64_t>(kTile, n - start), (int32_t)blockIdx.x, (int32_t)tiles};
+}
+
 // Digit histogram of one tile: per-wave LDS counters (a quarter of the
-// same-address contention), summed into counts[d * tiles + tile].
+// same-address contention), summed into counts[cbase + d * stride].
 template <typename K>
 __global__ __launch_bounds__(kBlock) void radix_count(const K *__restrict__ keys, int64_t n, int shift,
-                                                      int64_t tiles, int32_t *__restrict__ counts) {
+                                                      const RadixTile *__restrict__ table, int64_t tiles,
+                                                      int32_t *__restrict__ counts) {
   __shared__ int hist[kBlock / 64][256];
   const int w = threadIdx.x >> 6;
   for (int i = threadIdx.x; i < 256 * (kBlock / 64); i += kBlock) (&hist[0][0])[i] = 0;
   __syncthreads();
-  const int64_t base = (int64_t)blockIdx.x * kTile;
+  const RadixTile tl = tile_of(table, n, tiles);
 #pragma unroll 4
   for (int i = 0; i < kItems; ++i) {
-    const int64_t g = base + i * kBlock + threadIdx.x;
-    if (g < n) atomicAdd(&hist[w][(int)((keys[g] >> shift) & 255)], 1);
+    const int o = i * kBlock + (int)threadIdx.x;
+    if (o < tl.count) atomicAdd(&hist[w][(int)((keys[(int64_t)tl.start + o] >> shift) & 255)], 1);
   }
   __syncthreads();
   const int d = threadIdx.x;
-  counts[(int64_t)d * tiles + blockIdx.x] = hist[0][d] + hist[1][d] + hist[2][d] + hist[3][d];
+  counts[(int64_t)tl.cbase + (int64_t)d * tl.stride] = hist[0][d] + hist[1][d] + hist[2][d] + hist[3][d];
 }
 
-// Stable scatter of one tile by the digit at `shift`.  offsets[d * tiles + t]
-// is the first output position of tile t's keys with digit d.  Wave w ranks
+// Stable scatter of one tile by the digit at `shift`.  offsets[cbase + d *
+// stride] is the first output position of the tile's keys with digit d.  Wave w ranks
 // keys [w * 1024, (w + 1) * 1024) of the tile, 64 per step in key order: the
 // lanes holding the same digit find each other with eight ballots, take the
 // wave's running count of that digit, and the lowest of them advances it.
 template <typename K, bool kLast>
 __global__ __launch_bounds__(kBlock) void radix_scatter(const K *__restrict__ kin, const uint32_t *__restrict__ vin,
-                                                        int64_t n, int shift, int64_t tiles,
-                                                        const int32_t *__restrict__ offsets, K *__restrict__ kout,
-                                                        uint32_t *__restrict__ vout) {
+                                                        int64_t n, int shift, const RadixTile *__restrict__ table,
+                                                        int64_t tiles, const int32_t *__restrict__ offsets,
+                                                        K *__restrict__ kout, uint32_t *__restrict__ vout) {
   constexpr int kWaves = kBlock / 64;
   __shared__ int wcnt[kWaves][256];
   __shared__ int dstart[256];
@@ -179,22 +193,22 @@ __global__ __launch_bounds__(kBlock) void radix_scatter(const K *__restrict__ ki
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int i = threadIdx.x; i < 256 * kWaves; i += kBlock) (&wcnt[0][0])[i] = 0;
   __syncthreads();
-  const int64_t tile0 = (int64_t)blockIdx.x * kTile;
-  const int64_t wbase = tile0 + (int64_t)w * 64 * kItems;
+  const RadixTile tl = tile_of(table, n, tiles);
+  const int wbase = w * 64 * kItems;  // the wave's first key within the tile
   const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
   K key[kItems];
   uint32_t val[kItems];
   int rank[kItems];
 #pragma unroll
   for (int r = 0; r < kItems; ++r) {
-    const int64_t g = wbase + r * 64 + lane;
-    const bool ok = g < n;
-    key[r] = ok ? kin[g] : K(0);
-    val[r] = ok ? vin[g] : 0u;
+    const int o = wbase + r * 64 + lane;
+    const bool ok = o < tl.count;
+    key[r] = ok ? kin[(int64_t)tl.start + o] : K(0);
+    val[r] = ok ? vin[(int64_t)tl.start + o] : 0u;
   }
 #pragma unroll
   for (int r = 0; r < kItems; ++r) {
-    const bool ok = wbase + r * 64 + lane < n;
+    const bool ok = wbase + r * 64 + lane < tl.count;
     const int d = (int)((key[r] >> shift) & 255);
     uint64_t peers = __ballot(ok);
 #pragma unroll
@@ -227,7 +241,7 @@ __global__ __launch_bounds__(kBlock) void radix_scatter(const K *__restrict__ ki
       run += c[k];
     }
     dstart[d] = start;
-    gbase[d] = offsets[(int64_t)d * tiles + blockIdx.x];
+    gbase[d] = offsets[(int64_t)tl.cbase + (int64_t)d * tl.stride];
   }
   __syncthreads();
 #pragma unroll
@@ -238,7 +252,7 @@ __global__ __launch_bounds__(kBlock) void radix_scatter(const K *__restrict__ ki
       sval[pos] = val[r];
     }
   __syncthreads();
-  const int m = (int)min<int64_t>(kTile, n - tile0);
+  const int m = tl.count;
   for (int i = threadIdx.x; i < m; i += kBlock) {
     const K k = skey[i];
     const int d = (int)((k >> shift) & 255);
@@ -268,17 +282,31 @@ void exclusive_scan(const int32_t *in, int32_t *out, int64_t n, int32_t *temp, h
   FLK_HIP(hipGetLastError());
 }
 
-size_t radix_temp_bytes(int64_t n) {
-  const int64_t cells = 256 * tiles_of(std::max<int64_t>(n, 1));
+size_t radix_temp_bytes(int64_t n, int64_t table_tiles) {
+  const int64_t cells = 256 * std::max<int64_t>(tiles_of(std::max<int64_t>(n, 1)), table_tiles);
   return sizeof(int32_t) * (size_t)cells + scan_temp_bytes(cells);
+}
+
+int64_t segmented_tiles(const int64_t *seg_off, int64_t nseg, std::vector<RadixTile> &table) {
+  table.clear();
+  for (int64_t c = 0; c < nseg; ++c) {
+    const int64_t len = seg_off[c + 1] - seg_off[c];
+    const int64_t nt = (len + kTile - 1) / kTile;
+    const int64_t first = (int64_t)table.size();
+    for (int64_t j = 0; j < nt; ++j)
+      table.push_back(RadixTile{(int32_t)(seg_off[c] + j * kTile), (int32_t)std::min<int64_t>(kTile, len - j * kTile),
+                                (int32_t)(256 * first + j), (int32_t)nt});
+  }
+  return (int64_t)table.size();
 }
 
 template <typename K>
 void radix_sort_pairs(K *keys, K *keys_alt, uint32_t *vals, uint32_t *vals_alt, uint32_t *vals_out, int64_t n,
-                      int end_bit, void *temp, hipStream_t s) {
+                      int end_bit, void *temp, hipStream_t s, const RadixTile *table, int64_t table_tiles) {
   if (n <= 0) return;
   if (n >= (int64_t)INT32_MAX) throw Error(FLK_ERR_ARG, "radix sort: too many keys");
-  const int64_t tiles = tiles_of(n);
+  const int64_t tiles = table ? table_tiles : tiles_of(n);
+  if (tiles <= 0) return;
   const int64_t cells = 256 * tiles;
   int32_t *counts = static_cast<int32_t *>(temp);
   int32_t *part = counts + cells;
@@ -288,22 +316,23 @@ void radix_sort_pairs(K *keys, K *keys_alt, uint32_t *vals, uint32_t *vals_alt, 
   for (int p = 0; p < passes; ++p) {
     const K *ki = kbuf[p & 1];
     const uint32_t *vi = vbuf[p & 1];
-    hipLaunchKernelGGL(radix_count<K>, dim3((unsigned)tiles), dim3(kBlock), 0, s, ki, n, 8 * p, tiles, counts);
+    hipLaunchKernelGGL(radix_count<K>, dim3((unsigned)tiles), dim3(kBlock), 0, s, ki, n, 8 * p, table, tiles,
+                       counts);
     FLK_HIP(hipGetLastError());
     exclusive_scan(counts, counts, cells, part, s);
     if (p + 1 == passes)
       hipLaunchKernelGGL((radix_scatter<K, true>), dim3((unsigned)tiles), dim3(kBlock), 0, s, ki, vi, n, 8 * p,
-                         tiles, (const int32_t *)counts, (K *)nullptr, vals_out);
+                         table, tiles, (const int32_t *)counts, (K *)nullptr, vals_out);
     else
       hipLaunchKernelGGL((radix_scatter<K, false>), dim3((unsigned)tiles), dim3(kBlock), 0, s, ki, vi, n, 8 * p,
-                         tiles, (const int32_t *)counts, kbuf[(p + 1) & 1], vbuf[(p + 1) & 1]);
+                         table, tiles, (const int32_t *)counts, kbuf[(p + 1) & 1], vbuf[(p + 1) & 1]);
     FLK_HIP(hipGetLastError());
   }
 }
 
 template void radix_sort_pairs<uint32_t>(uint32_t *, uint32_t *, uint32_t *, uint32_t *, uint32_t *, int64_t, int,
-                                         void *, hipStream_t);
+                                         void *, hipStream_t, const RadixTile *, int64_t);
 template void radix_sort_pairs<uint64_t>(uint64_t *, uint64_t *, uint32_t *, uint32_t *, uint32_t *, int64_t, int,
-                                         void *, hipStream_t);
+                                         void *, hipStream_t, const RadixTile *, int64_t);
 
 }  // namespace flk
