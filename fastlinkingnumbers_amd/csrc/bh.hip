@@ -102,21 +102,14 @@ __global__ __launch_bounds__(256) void bh_keys(const double4 *__restrict__ pts, 
   }
 }
 
-// Splits every node of one level with more than `leaf` segments: children get
-// slots 2*scan[k], 2*scan[k]+1 of the next level.
-__global__ __launch_bounds__(256) void bh_split_flags(const BhRange *__restrict__ lev, int32_t n,
-                                                      int32_t leaf, int32_t *__restrict__ flag) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  flag[k] = (lev[k].hi - lev[k].lo) > leaf ? 1 : 0;
-}
-
-__global__ __launch_bounds__(256) void bh_split(const BhRange *__restrict__ lev, int32_t n,
-                                                int32_t base, int32_t next_base,
-                                                const int32_t *__restrict__ flag,
-                                                const int32_t *__restrict__ scan,
-                                                BhRange *__restrict__ next,
-                                                int2 *__restrict__ child) {
+// Splits every node of one level with more than `leaf` segments (flag[k],
+// written by the previous level's split, or from the host for the roots):
+// children get slots 2*scan[k], 2*scan[k]+1 of the next level, and their own
+// split flags in next_flag.
+__global__ __launch_bounds__(256) void bh_split(const BhRange *__restrict__ lev, int32_t n, int32_t base,
+                                                int32_t next_base, int32_t leaf, const int32_t *__restrict__ flag,
+                                                const int32_t *__restrict__ scan, BhRange *__restrict__ next,
+                                                int2 *__restrict__ child, int32_t *__restrict__ next_flag) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const BhRange r = lev[k];
@@ -125,6 +118,8 @@ __global__ __launch_bounds__(256) void bh_split(const BhRange *__restrict__ lev,
     const int mid = r.lo + (r.hi - r.lo) / 2;
     next[s] = BhRange{r.lo, mid};
     next[s + 1] = BhRange{mid, r.hi};
+    next_flag[s] = mid - r.lo > leaf ? 1 : 0;
+    next_flag[s + 1] = r.hi - mid > leaf ? 1 : 0;
     child[base + k] = make_int2(next_base + s, next_base + s + 1);
   } else {
     child[base + k] = make_int2(-1, -1);
@@ -1187,13 +1182,15 @@ struct ForestPlan {
   std::vector<BhRange> roots;
   std::vector<RadixTile> tiles;  // the segmented sort's tiles (one curve each)
   std::vector<int32_t> tcurve;   // each tile's curve rank
+  std::vector<int32_t> root_flag;  // the roots' split flags (more than `leaf` segments)
   int64_t S = 0;
   std::vector<int32_t> level_base, level_n;
   std::vector<int64_t> level_split, level_max;
   int32_t total = 0;
   int lev_top = 0;
-  // In a graph: seg_off, vtx_off, cbox, roots, tiles and tcurve uploaded once
-  // (dsrc, in that order), copied device to device on every replay.
+  // In a graph: seg_off, vtx_off, cbox, roots, tiles, tcurve and root_flag
+  // uploaded once (dsrc, in that order), copied device to device on every
+  // replay.
   DevBuf dconst;
   std::vector<const void *> dsrc;
 };
@@ -1256,6 +1253,8 @@ void plan_forest(flk_ctx *ctx, const std::vector<int32_t> &curves_in, int leaf, 
   if (S >= INT32_MAX / 2) throw Error(FLK_ERR_ARG, "too many Barnes-Hut segments");
   P.roots.resize(nc);
   for (int32_t k = 0; k < nc; ++k) P.roots[k] = BhRange{(int32_t)P.seg_off[k], (int32_t)P.seg_off[k + 1]};
+  P.root_flag.resize(nc);
+  for (int32_t k = 0; k < nc; ++k) P.root_flag[k] = P.seg_off[k + 1] - P.seg_off[k] > leaf ? 1 : 0;
   const int64_t ntiles = segmented_tiles(P.seg_off.data(), nc, P.tiles);
   P.tcurve.resize(ntiles);
   for (int32_t k = 0, t = 0; k < nc; ++k)
@@ -1325,6 +1324,7 @@ void plan_forest(flk_ctx *ctx, const std::vector<int32_t> &curves_in, int leaf, 
   B.ids0.reserve(sizeof(uint32_t) * S);
   B.order.reserve(sizeof(uint32_t) * S);
   B.flag.reserve(sizeof(int32_t) * std::max<int64_t>(S + nc + 1, widest));
+  B.flag2.reserve(sizeof(int32_t) * std::max<int64_t>(S + nc + 1, widest));
   B.scan.reserve(sizeof(int32_t) * std::max<int64_t>(S + nc + 1, widest));  // also the sort's value scratch
   B.tmp.reserve(std::max(radix_temp_bytes(S, ntiles), scan_temp_bytes(widest)));
   B.sp.reserve(sizeof(double2) * 3 * S);
@@ -1346,19 +1346,18 @@ void enqueue_moments(flk_ctx *ctx, const ForestPlan &P, SP *sp, hipStream_t s) {
   hipLaunchKernelGGL(bh_stage_segments<SP>, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, s,
                      B.order.as<uint32_t>(), cv.dpts.as<double4>(), S, sp);
   FLK_HIP(hipGetLastError());
-  // level-by-level node creation
+  // level-by-level node creation; split flags alternate between two buffers
   put(B.range.ptr, P.roots.data(), P.dsrc, 3, sizeof(BhRange) * nc, s);
+  put(B.flag.ptr, P.root_flag.data(), P.dsrc, 6, sizeof(int32_t) * nc, s);
+  int32_t *flags[2] = {B.flag.as<int32_t>(), B.flag2.as<int32_t>()};
   for (size_t lev = 0; lev < P.level_base.size(); ++lev) {
     const int32_t base = P.level_base[lev], n = P.level_n[lev];
     const int32_t next = lev + 1 < P.level_base.size() ? P.level_base[lev + 1] : P.total;
     const unsigned nb = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL(bh_split_flags, dim3(nb), dim3(256), 0, s, B.range.as<BhRange>() + base, n, P.leaf,
-                       B.flag.as<int32_t>());
-    FLK_HIP(hipGetLastError());
-    exclusive_scan(B.flag.as<int32_t>(), B.scan.as<int32_t>(), n, B.tmp.as<int32_t>(), s);
-    hipLaunchKernelGGL(bh_split, dim3(nb), dim3(256), 0, s, B.range.as<BhRange>() + base, n, base, next,
-                       B.flag.as<int32_t>(), B.scan.as<int32_t>(), B.range.as<BhRange>() + next,
-                       B.child.as<int2>());
+    exclusive_scan(flags[lev & 1], B.scan.as<int32_t>(), n, B.tmp.as<int32_t>(), s);
+    hipLaunchKernelGGL(bh_split, dim3(nb), dim3(256), 0, s, B.range.as<BhRange>() + base, n, base, next, P.leaf,
+                       (const int32_t *)flags[lev & 1], (const int32_t *)B.scan.as<int32_t>(),
+                       B.range.as<BhRange>() + next, B.child.as<int2>(), flags[(lev + 1) & 1]);
     FLK_HIP(hipGetLastError());
   }
   if (P.lev_top < (int)P.level_base.size()) {
@@ -1863,7 +1862,8 @@ struct BatchPlan {
                                {forest.cbox.data(), 8 * forest.cbox.size()},
                                {forest.roots.data(), sizeof(BhRange) * forest.roots.size()},
                                {forest.tiles.data(), sizeof(RadixTile) * forest.tiles.size()},
-                               {forest.tcurve.data(), 4 * forest.tcurve.size()}},
+                               {forest.tcurve.data(), 4 * forest.tcurve.size()},
+                               {forest.root_flag.data(), 4 * forest.root_flag.size()}},
                               s);
   }
   void collect() {  // pinned read-back into the vectors

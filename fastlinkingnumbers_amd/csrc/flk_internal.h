@@ -193,7 +193,7 @@ struct HostStage {
 // FP64 build records and the traversal queues.
 struct BhScratch {
   // forest
-  DevBuf geo, mom, order, m64, range, child, flag, scan, keys0, keys1, ids0, tmp, seg, vtx, box, sp, rtiles, rcurve;
+  DevBuf geo, mom, order, m64, range, child, flag, flag2, scan, keys0, keys1, ids0, tmp, seg, vtx, box, sp, rtiles, rcurve;
   // traversal: root items of a pass, re-run list, per-pair sums and results
   DevBuf roots_a, roots_b, beta2, cnt, leaves, pred, sum, err, items0, redo, outv, oute, outf;
   std::vector<std::unique_ptr<DevBuf>> levels;
@@ -208,7 +208,7 @@ struct BhScratch {
     return *levels.back();
   }
   template <typename Fn> void each(Fn fn) {
-    DevBuf *all[] = {&geo, &mom, &order, &m64, &range, &child, &flag, &scan, &keys0, &keys1, &ids0,
+    DevBuf *all[] = {&geo, &mom, &order, &m64, &range, &child, &flag, &flag2, &scan, &keys0, &keys1, &ids0,
                      &tmp, &seg, &vtx, &box, &sp, &rtiles, &rcurve, &roots_a, &roots_b, &beta2, &cnt, &leaves, &pred,
                      &sum, &err, &items0, &redo, &outv, &oute, &outf};
     for (DevBuf *b : all) fn(*b);

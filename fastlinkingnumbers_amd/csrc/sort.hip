@@ -152,21 +152,33 @@ __device__ __forceinline__ RadixTile tile_of(const RadixTile *__restrict__ table
   return RadixTile{(int32_t)start, (int32_t)min<int64_t>(kTile, n - start), (int32_t)blockIdx.x, (int32_t)tiles};
 }
 
-// Digit histogram of one tile: per-wave LDS counters (a quarter of the
-// same-address contention), summed into counts[cbase + d * stride].
+// Digit histogram of one tile: per-wave LDS counters, summed into
+// counts[cbase + d * stride].  A wave reads 64 consecutive keys per step and
+// counts runs of equal digits in consecutive lanes with one LDS add per run:
+// keys that arrive nearly sorted (the high digits of Morton codes along a
+// curve) would otherwise serialise on one counter.
 template <typename K>
 __global__ __launch_bounds__(kBlock) void radix_count(const K *__restrict__ keys, int64_t n, int shift,
                                                       const RadixTile *__restrict__ table, int64_t tiles,
                                                       int32_t *__restrict__ counts) {
   __shared__ int hist[kBlock / 64][256];
-  const int w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int i = threadIdx.x; i < 256 * (kBlock / 64); i += kBlock) (&hist[0][0])[i] = 0;
   __syncthreads();
   const RadixTile tl = tile_of(table, n, tiles);
+  const uint64_t above = lane == 63 ? 0ull : (~0ull << (lane + 1));
 #pragma unroll 4
   for (int i = 0; i < kItems; ++i) {
     const int o = i * kBlock + (int)threadIdx.x;
-    if (o < tl.count) atomicAdd(&hist[w][(int)((keys[(int64_t)tl.start + o] >> shift) & 255)], 1);
+    const bool ok = o < tl.count;
+    const int d = ok ? (int)((keys[(int64_t)tl.start + o] >> shift) & 255) : -1;
+    const int dp = __shfl_up(d, 1, 64);
+    // a run starts at lane 0 and wherever the digit changes; invalid lanes
+    // (the tile's tail, always the last lanes) form a run that is not added
+    const uint64_t starts = __ballot(lane == 0 || d != dp);
+    const uint64_t later = starts & above;
+    const int next = later ? __builtin_ctzll(later) : 64;
+    if (ok && (lane == 0 || d != dp)) atomicAdd(&hist[w][d], next - lane);
   }
   __syncthreads();
   const int d = threadIdx.x;
