@@ -135,6 +135,9 @@ def main():
                          "(default 4096^2 for workload 4, else 0)")
     ap.add_argument("--criterion", default="beta", choices=["beta", "mlp"],
                     help="Barnes-Hut opening criterion for --method barneshut")
+    ap.add_argument("--graph", type=int, default=1,
+                    help="1: each certificate is captured once and replayed as a HIP graph; 0: launched eagerly "
+                         "(per-kernel traces, the build / traversal split)")
     ap.add_argument("--fp64-check", action="store_true",
                     help="FP64 direct sum of the workload-5 pair against every estimate (one JSON line)")
     ap.add_argument("--cpu-worker", default=None, help=argparse.SUPPRESS)
@@ -181,7 +184,7 @@ def main():
     if args.direct_threshold is None:
         args.direct_threshold = 4096 * 4096 if args.workload == 4 else 0
     params = flk.default_params(method=args.method, criterion=args.criterion,
-                                bh_direct_threshold=args.direct_threshold)
+                                bh_direct_threshold=args.direct_threshold, graph=args.graph)
     if args.criterion == "mlp":
         from fastlinkingnumbers_amd import criterion
         eng.mlp_set_weights(*criterion.load_weights())
@@ -304,6 +307,7 @@ def main():
             "max_off_integer": float(off_int.max()) if len(off_int) else 0.0,
             "fallbacks": int(st["fallbacks"]),
             "direct_threshold": args.direct_threshold,
+            "graph": args.graph,
             "ds_seg_pairs": int(st["ds_seg_pairs"]),
             "bh_visits": int(st["bh_visits"]),
         },
