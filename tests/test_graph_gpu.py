@@ -126,3 +126,51 @@ def test_tree_order_is_the_stable_sort_of_morton_keys(engine):
     expect = np.argsort(key, kind="stable")
     assert len(np.unique(key)) < len(key) - 500  # many ties
     assert np.array_equal(order, expect)
+
+
+def test_graph_fp64_direct_part_and_judge(engine):
+    """The hybrid graph with its direct-sum part in FP64, and the FP64 judge
+    after a graph launch: a fallback tolerance below the Barnes–Hut error
+    sends the Barnes–Hut pairs to the FP64 direct sum after the graph, whose
+    results must equal a plain FP64 direct sum of the same pairs."""
+    curves = W.workload(4, scale=0.01)
+    engine.set_curves(curves)
+    pairs = engine.potential_links()
+    kw = dict(method="barneshut", bh_direct_threshold=512 * 512)
+    eager = engine.linking_numbers(pairs, graph=0, fp64=1, fallback_tol=0.0, **kw)
+    g = engine.linking_numbers(pairs, fp64=1, fallback_tol=0.0, **kw)
+    assert g.stats["graph_launches"] == 1
+    _same(g, eager)
+    ds64 = engine.linking_numbers(pairs, method="directsum", fp64=1, fallback_tol=0.0)
+    direct = (g.flags & flk.FLK_PAIR_DIRECT) != 0
+    assert direct.any() and (~direct).any()
+    # the direct-sum part of the graph is the FP64 direct sum (its tasks split
+    # the pairs by the batch's work, so the sums agree to rounding)
+    np.testing.assert_allclose(g.lk[direct], ds64.lk[direct], rtol=0, atol=1e-12)
+    judged = engine.linking_numbers(pairs, fallback_tol=1e-12, **kw)
+    assert judged.stats["graph_launches"] == 1
+    redone = (judged.flags & flk.FLK_PAIR_FALLBACK) != 0
+    assert redone.sum() == judged.stats["fallbacks"] and redone[~direct].all()
+    np.testing.assert_allclose(judged.lk[redone], ds64.lk[redone], rtol=0, atol=1e-9)
+
+
+def test_graph_recaptured_after_scratch_reallocation(engine):
+    """A certificate graph holds pointers into the Barnes–Hut scratch.  A call
+    that grows that scratch (a tree export of a larger curve set) must make
+    the next certificate capture a new graph rather than replay the old one;
+    the results stay those of the eager launch."""
+    small = W.random_walk_pair(seed=4, n=4000, rg=15.0, workload=3)
+    engine.set_curves(small)
+    kw = dict(method="barneshut", fallback_tol=0.0)
+    eager = engine.linking_numbers([[0, 1]], graph=0, **kw)
+    first = engine.linking_numbers([[0, 1]], **kw)
+    _same(first, eager)
+    big = W.random_walk_pair(seed=5, n=200000, rg=60.0, workload=3)
+    engine.set_curves(small + big)
+    engine.tree_moments(2, leaf_size=1)  # grows the tree buffers
+    again = engine.linking_numbers([[0, 1]], **kw)
+    assert again.stats["graph_launches"] == 1
+    _same(again, eager)
+    both = engine.linking_numbers([[0, 1], [2, 3]], **kw)
+    eager_both = engine.linking_numbers([[0, 1], [2, 3]], graph=0, **kw)
+    _same(both, eager_both)
