@@ -777,16 +777,23 @@ __global__ __launch_bounds__(kTravBlock) __attribute__((amdgpu_waves_per_eu(4)))
     int ca = 0, cb = 0;
     bool split_a = false;
     float pv = __builtin_inff();  // learned criterion: predicted log10 far-field error
+    // Learned criterion: both node records are loaded once, before the
+    // prediction, and kept for the classification (a dead lane reads node 0
+    // of its empty item; loading them again after the network's LDS fences
+    // cost another round trip: 2x8M learned 11.75 -> 11.15 ms).  Beta loads
+    // them inside the live branch (loading them before it measured 4% slower).
+    BhGeo A0, B0;
     if constexpr (MLP) {
-      const BhGeo A = geo[it.a], B = geo[it.b];
-      const float rx = (B.cx - A.cx) + (B.lx - A.lx), ry = (B.cy - A.cy) + (B.ly - A.ly),
-                  rz = (B.cz - A.cz) + (B.lz - A.lz);
+      A0 = geo[it.a];
+      B0 = geo[it.b];
+      const float rx = (B0.cx - A0.cx) + (B0.lx - A0.lx), ry = (B0.cy - A0.cy) + (B0.ly - A0.ly),
+                  rz = (B0.cz - A0.cz) + (B0.lz - A0.lz);
       const float dsq = fmaf(rx, rx, fmaf(ry, ry, rz * rz)) + eps;
-      const float rs = A.radius + B.radius;
-      pv = wave_predict(live && !(A.c0 < 0 && B.c0 < 0) && dsq > rs * rs + eps, A, B, dsq, ms, wave_evals);
+      const float rs = A0.radius + B0.radius;
+      pv = wave_predict(live && !(A0.c0 < 0 && B0.c0 < 0) && dsq > rs * rs + eps, A0, B0, dsq, ms, wave_evals);
     }
     if (live) {
-      const BhGeo A = geo[it.a], B = geo[it.b];
+      const BhGeo A = MLP ? A0 : geo[it.a], B = MLP ? B0 : geo[it.b];
       const F3 r = f3((B.cx - A.cx) + (B.lx - A.lx), (B.cy - A.cy) + (B.ly - A.ly),
                       (B.cz - A.cz) + (B.lz - A.lz));
       const float dsq = dot3(r, r) + eps;
