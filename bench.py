@@ -91,6 +91,7 @@ def cpu_worker(spec):
             t = time.perf_counter()
             trip, _, _, cand, lk = m.certificate("barneshut", discretize=False)
             times.append(time.perf_counter() - t)
+            log(f"cpu certificate run {len(times)}: {times[-1]:.2f} s")
         out = {"triplets": [list(map(int, x)) for x in trip], "lk": [float(v) for v in lk],
                "pairs": np.asarray(cand).tolist()}
     else:
@@ -100,6 +101,7 @@ def cpu_worker(spec):
             t = time.perf_counter()
             lk = O.ds_pairs(xyz, offs, sel)
             times.append(time.perf_counter() - t)
+            log(f"cpu direct-sum run {len(times)}: {times[-1]:.2f} s")
         out = {"lk": [float(v) for v in lk]}
     out["times"] = times
     print(json.dumps(out), flush=True)
@@ -107,10 +109,11 @@ def cpu_worker(spec):
 
 def run_cpu_child(spec, timeout=600):
     env = dict(os.environ, OMP_PROC_BIND="spread", OMP_PLACES="cores")
+    # the child's progress lines go straight to this process's stderr
     r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-worker", json.dumps(spec)],
-                       capture_output=True, text=True, env=env, timeout=timeout)
+                       stdout=subprocess.PIPE, text=True, env=env, timeout=timeout)
     if r.returncode != 0:
-        raise RuntimeError("cpu baseline child failed: " + r.stderr[-2000:])
+        raise RuntimeError(f"cpu baseline child failed with status {r.returncode} (its stderr is above)")
     return json.loads(r.stdout.strip().splitlines()[-1])
 
 
@@ -427,10 +430,46 @@ def learned_vs_directsum(eng, flk, workloads, args):
         log(f"workload 5 {name}: {res[name]}")
     res["learned_speedup_vs_ds"] = round(res["ds_ms"] / res["learned"]["ms"], 1)
     res["learned_speedup_vs_beta"] = round(res["beta"]["ms"] / res["learned"]["ms"], 2)
+    if args.cpu_seconds > 0:
+        res["cpu_reference_bh"] = cpu_bh_reference(eng, workloads)
     res["same_integer"] = round(res["beta"]["lk"]) == round(res["learned"]["lk"])
     if "ds_lk" in res:
         res["same_integer_as_ds"] = round(res["ds_lk"]) == round(res["learned"]["lk"]) == round(res["beta"]["lk"])
     return res
+
+
+def cpu_bh_reference(eng, workloads):
+    """The reference's own Barnes-Hut certificate of the 2 x 8M pair on the
+    host (the oracle's restatement of ComputeFromModel: potential links,
+    within-pair BFS then DFS under OpenMP, adaptive beta, rounding), median
+    of 3 runs, beside the GPU beta and learned certificates of the same pair
+    timed the same way (potential links, then the pair, wall clock)."""
+    model, threads = cpu_info()
+    gpu = {}
+    for name, kw in (("beta", {}), ("learned", {"criterion": "mlp"})):
+        eng.linking_numbers(eng.potential_links(), method="barneshut", **kw)
+        ms = []
+        for _ in range(3):
+            t = time.perf_counter()
+            pairs = eng.potential_links()
+            r = eng.linking_numbers(pairs, method="barneshut", **kw)
+            ms.append((time.perf_counter() - t) * 1e3)
+        gpu[name] = {"ms": round(statistics.median(ms), 2), "lk": float(r.lk[0]), "pairs": len(pairs)}
+    log("workload 5: CPU Barnes-Hut certificate, 3 runs")
+    r = run_cpu_child({"kind": "certificate", "workload": 5, "scale": 1.0, "reps": 3}, timeout=900)
+    dt = statistics.median(r["times"])
+    cpu_lk = r["lk"][0] if r["lk"] else None
+    out = {"kind": "port", "cores": threads, "cpu_model": model, "runs": 3,
+           "binding": "OMP_PROC_BIND=spread OMP_PLACES=cores", "times_s": [round(x, 3) for x in r["times"]],
+           "certificate_s": round(dt, 3), "lk": cpu_lk, "gpu_whole_certificate": gpu,
+           "same_integer": cpu_lk is not None and
+           round(cpu_lk) == round(gpu["beta"]["lk"]) == round(gpu["learned"]["lk"]),
+           "speedup_beta": round(dt * 1e3 / gpu["beta"]["ms"], 1),
+           "speedup_learned": round(dt * 1e3 / gpu["learned"]["ms"], 1),
+           "sample": "whole certificate of the 2 x 8M random-walk pair through the oracle's restatement of "
+                     "ComputeFromModel"}
+    log(f"workload 5 CPU Barnes-Hut: {out}")
+    return out
 
 
 def fan_pieces(c, k):

@@ -47,6 +47,10 @@ namespace flk {
 struct BhRange {    // node range during construction
   int32_t lo, hi;
 };
+// Nodes of at most this many segments take their moments from their segments
+// (no FP64 record round trip; a node reads at most 2 x 8 segments).
+constexpr int32_t kDirectSegments = 8;
+
 struct Mom64 {      // FP64 moments during construction
   double box[6];
   double c[3];
@@ -105,7 +109,8 @@ __global__ __launch_bounds__(256) void bh_keys(const double4 *__restrict__ pts, 
 // Splits every node of one level with more than `leaf` segments (flag[k],
 // written by the previous level's split, or from the host for the roots):
 // children get slots 2*scan[k], 2*scan[k]+1 of the next level, and their own
-// split flags in next_flag.
+// split flags in next_flag.  scan is null on a level where every node splits
+// (the exclusive scan of all ones is k).
 __global__ __launch_bounds__(256) void bh_split(const BhRange *__restrict__ lev, int32_t n, int32_t base,
                                                 int32_t next_base, int32_t leaf, const int32_t *__restrict__ flag,
                                                 const int32_t *__restrict__ scan, BhRange *__restrict__ next,
@@ -114,7 +119,7 @@ __global__ __launch_bounds__(256) void bh_split(const BhRange *__restrict__ lev,
   if (k >= n) return;
   const BhRange r = lev[k];
   if (flag[k]) {
-    const int s = 2 * scan[k];
+    const int s = 2 * (scan ? scan[k] : k);
     const int mid = r.lo + (r.hi - r.lo) / 2;
     next[s] = BhRange{r.lo, mid};
     next[s + 1] = BhRange{mid, r.hi};
@@ -157,21 +162,45 @@ __device__ __forceinline__ void seg_ends(const SP *__restrict__ sp, int s, doubl
   pb[0] = v.y; pb[1] = w.x; pb[2] = w.y;
 }
 
-// Bounding box of the segments [r.lo, r.hi) of a leaf bucket.
+// f(pa, pb) for the segments [lo, lo + cnt) in order, their end points read
+// kSegChunk segments at a time with every load of a chunk issued before the
+// first is used (one round trip per chunk, not per segment).
+constexpr int kSegChunk = 4;
+template <typename SP, class F>
+__device__ __forceinline__ void each_seg(const SP *__restrict__ sp, int lo, int cnt, F &&f) {
+  for (int q0 = 0; q0 < cnt; q0 += kSegChunk) {
+    const SP *p = sp + 3 * ((int64_t)lo + q0);
+    SP v[3 * kSegChunk];
+#pragma unroll
+    for (int q = 0; q < kSegChunk; ++q)
+      if (q0 + q < cnt) {
+        v[3 * q] = p[3 * q];
+        v[3 * q + 1] = p[3 * q + 1];
+        v[3 * q + 2] = p[3 * q + 2];
+      }
+#pragma unroll
+    for (int q = 0; q < kSegChunk; ++q)
+      if (q0 + q < cnt) {
+        double pa[3] = {(double)v[3 * q].x, (double)v[3 * q].y, (double)v[3 * q + 1].x};
+        double pb[3] = {(double)v[3 * q + 1].y, (double)v[3 * q + 2].x, (double)v[3 * q + 2].y};
+        f(pa, pb);
+      }
+  }
+}
+
+// Bounding box of the segments [r.lo, r.hi) of a small node.
 template <typename SP>
 __device__ void leaf_box(BhRange r, const SP *__restrict__ sp, double (&box)[6]) {
   for (int t = 0; t < 3; ++t) {
     box[t] = DBL_MAX;
     box[3 + t] = -DBL_MAX;
   }
-  for (int s = r.lo; s < r.hi; ++s) {
-    double pa[3], pb[3];
-    seg_ends(sp, s, pa, pb);
+  each_seg(sp, r.lo, r.hi - r.lo, [&](const double (&pa)[3], const double (&pb)[3]) {
     for (int d = 0; d < 3; ++d) {
       box[d] = fmin(box[d], fmin(pa[d], pb[d]));
       box[3 + d] = fmax(box[3 + d], fmax(pa[d], pb[d]));
     }
-  }
+  });
 }
 
 // FP64 moments -> FP32 node records (centre hi/lo, radius, children, segment
@@ -231,9 +260,7 @@ __device__ __forceinline__ void direct_records(BhRange rr, int2 ch, const SP *__
   for (int t = 0; t < 3; ++t) m.c[t] = 0.f;
   for (int t = 0; t < 9; ++t) m.D[t] = 0.f;
   for (int t = 0; t < 18; ++t) m.Q[t] = 0.f;
-  for (int s = rr.lo; s < rr.hi; ++s) {
-    double pa[3], pb[3];
-    seg_ends(sp, s, pa, pb);
+  each_seg(sp, rr.lo, rr.hi - rr.lo, [&](const double (&pa)[3], const double (&pb)[3]) {
     float t[3], w[3];
     for (int d = 0; d < 3; ++d) {
       t[d] = (float)(pb[d] - pa[d]);
@@ -248,7 +275,7 @@ __device__ __forceinline__ void direct_records(BhRange rr, int2 ch, const SP *__
         for (int j = i; j < 3; ++j)
           m.Q[6 * kk + qidx(i, j)] =
               fmaf(t[kk], fmaf((1.0f / 12.0f) * t[i], t[j], w[i] * w[j]), m.Q[6 * kk + qidx(i, j)]);
-  }
+  });
   m.pad[0] = m.pad[1] = 0.f;
   const float hx = (float)ctr[0], hy = (float)ctr[1], hz = (float)ctr[2];
   gr.cx = hx; gr.cy = hy; gr.cz = hz;
@@ -683,8 +710,10 @@ struct NoMlpShared {};
 // Predictions of the learned criterion for the wave's 64 items (one per
 // lane; log10 of the far-field error, +inf where the decision does not
 // depend on it): the features of the items that need the network are
-// compacted into the wave's LDS rows and evaluated 16 per tile, and each lane
-// reads its item's prediction back.  Wave-uniform: every lane calls it.
+// compacted into the wave's LDS rows and evaluated 16 per tile, all of the
+// wave's tiles through each weight fragment read from LDS (infer_tiles), and
+// each lane reads its item's prediction back.  Wave-uniform: every lane calls
+// it.
 __device__ __forceinline__ float wave_predict(bool need, const BhGeo &A, const BhGeo &B, float dsq, MlpShared &ms,
                                               unsigned int &wave_evals) {
   const unsigned long long m = __ballot(need);
@@ -704,17 +733,23 @@ __device__ __forceinline__ float wave_predict(bool need, const BhGeo &A, const B
     *reinterpret_cast<f16x8 *>(&ms.xs[w][pos][8]) = v1;
   }
   wave_lds_sync();
-  const int cnt = __popcll(m);
+  const int cnt = __popcll(m), nt = (cnt + 15) >> 4;
   wave_evals += (unsigned int)cnt;
-  for (int r0 = 0; r0 < cnt; r0 += 16) {
-    const int row = r0 + c;
-    f16x8 bx;
-    if (g < 2 && row < cnt) bx = *reinterpret_cast<const f16x8 *>(&ms.xs[w][row][8 * g]);
+  f16x8 bx[4];
+  float yv[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int row = 16 * t + c;
+    if (t < nt && g < 2 && row < cnt) bx[t] = *reinterpret_cast<const f16x8 *>(&ms.xs[w][row][8 * g]);
     else
 #pragma unroll
-      for (int j = 0; j < 8; ++j) bx[j] = (_Float16)0.f;
-    const float yv = infer_tile(LdsWeights{(LdsW *)&ms.W, l}, ms.L, bx);
-    if (g == 0 && row < cnt) ms.ys[w][row] = yv;
+      for (int j = 0; j < 8; ++j) bx[t][j] = (_Float16)0.f;
+  }
+  infer_tiles<4>(LdsWeights{(LdsW *)&ms.W, l}, ms.L, bx, nt, yv);
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int row = 16 * t + c;
+    if (t < nt && g == 0 && row < cnt) ms.ys[w][row] = yv[t];
   }
   wave_lds_sync();
   const float p = need ? ms.ys[w][pos] : __builtin_inff();
@@ -728,7 +763,8 @@ __device__ __forceinline__ float wave_predict(bool need, const BhGeo &A, const B
 // the block has classified its 256 * IPT items the stage is copied out
 // coalesced behind one queue atomic each.  MLP: the learned criterion, its
 // network evaluated here on the node records the level already holds (the
-// weights in VGPRs, loaded once per block); *evals counts the evaluations.
+// weight fragments in LDS, loaded once per block); *evals counts the
+// evaluations.
 template <int IPT, bool MLP>
 __global__ __launch_bounds__(kTravBlock) __attribute__((amdgpu_waves_per_eu(4))) void bh_level(
     const BhItem *__restrict__ items, int64_t n, const unsigned long long *__restrict__ n_dev,
@@ -1163,9 +1199,6 @@ __global__ void bh_merge_redo(int64_t nb, const unsigned long long *__restrict__
 namespace {
 
 // Node records of the trees of one call; the buffers are the context's.
-// Nodes of at most this many segments take their moments from their segments
-// (no FP64 record round trip; a node reads at most 2 x 8 segments).
-constexpr int32_t kDirectSegments = 8;
 
 struct Forest {
   int32_t nnodes = 0;
@@ -1361,9 +1394,12 @@ void enqueue_moments(flk_ctx *ctx, const ForestPlan &P, SP *sp, hipStream_t s) {
     const int32_t base = P.level_base[lev], n = P.level_n[lev];
     const int32_t next = lev + 1 < P.level_base.size() ? P.level_base[lev + 1] : P.total;
     const unsigned nb = (unsigned)((n + 255) / 256);
-    exclusive_scan(flags[lev & 1], B.scan.as<int32_t>(), n, B.tmp.as<int32_t>(), s);
+    // slots of the split nodes: a scan of the flags, unless the plan shows
+    // that every node of the level splits
+    const bool all = P.level_split[lev] == n;
+    if (!all) exclusive_scan(flags[lev & 1], B.scan.as<int32_t>(), n, B.tmp.as<int32_t>(), s);
     hipLaunchKernelGGL(bh_split, dim3(nb), dim3(256), 0, s, B.range.as<BhRange>() + base, n, base, next, P.leaf,
-                       (const int32_t *)flags[lev & 1], (const int32_t *)B.scan.as<int32_t>(),
+                       (const int32_t *)flags[lev & 1], all ? nullptr : (const int32_t *)B.scan.as<int32_t>(),
                        B.range.as<BhRange>() + next, B.child.as<int2>(), flags[(lev + 1) & 1]);
     FLK_HIP(hipGetLastError());
   }

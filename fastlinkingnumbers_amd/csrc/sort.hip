@@ -28,7 +28,7 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kItems = 16;
 constexpr int kTile = kBlock * kItems;      // values per block tile
-constexpr int64_t kSingleBlockMax = 65536;  // shorter inputs: one block
+constexpr int64_t kSingleBlockMax = 4 * kTile;  // shorter inputs: one block
 
 __device__ __forceinline__ void wave_lds_order() {
   __builtin_amdgcn_wave_barrier();
@@ -80,64 +80,89 @@ __global__ __launch_bounds__(kBlock) void scan_tile_sums(const int32_t *__restri
   if (threadIdx.x == 0) part[blockIdx.x] = tot;
 }
 
-// One block, exclusive scan of in[0, n) into out (may alias), chunk by chunk
-// with a running carry; each thread owns 16 consecutive values of a chunk.
+// A tile of m <= kTile values into LDS (zeros past m), every load of a
+// thread issued before the first is waited for.
+__device__ __forceinline__ void tile_to_lds(const int32_t *in, int m, int32_t *buf) {
+  int v[kItems];
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    const int k = i * kBlock + threadIdx.x;
+    v[i] = k < m ? in[k] : 0;
+  }
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) buf[i * kBlock + threadIdx.x] = v[i];
+}
+
+// Exclusive scan of the LDS tile in place, plus carry; *tot receives the
+// tile's sum.  Each thread owns 16 consecutive values (four 16-byte LDS
+// accesses).  Contains block barriers.
+__device__ __forceinline__ void scan_lds_tile(int32_t *buf, int carry, int *tot, int *sw) {
+  int4 *b4 = reinterpret_cast<int4 *>(buf) + threadIdx.x * (kItems / 4);
+  int v[kItems], s = 0;
+#pragma unroll
+  for (int q = 0; q < kItems / 4; ++q) {
+    const int4 x = b4[q];
+    v[4 * q] = x.x;
+    v[4 * q + 1] = x.y;
+    v[4 * q + 2] = x.z;
+    v[4 * q + 3] = x.w;
+  }
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) s += v[i];
+  int run = carry + block_exclusive(s, tot, sw);
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    const int x = v[i];
+    v[i] = run;
+    run += x;
+  }
+#pragma unroll
+  for (int q = 0; q < kItems / 4; ++q) b4[q] = make_int4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+}
+
+__device__ __forceinline__ void lds_to_tile(const int32_t *buf, int m, int32_t *out) {
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    const int k = i * kBlock + threadIdx.x;
+    if (k < m) out[k] = buf[k];
+  }
+}
+
+// One block, exclusive scan of in[0, n) into out (may alias), tile by tile
+// with a running carry.
 __global__ __launch_bounds__(kBlock) void scan_one_block(const int32_t *in, int32_t *out, int64_t n) {
   __shared__ int sw[4];
-  __shared__ int32_t buf[kTile];
+  __shared__ int4 buf4[kTile / 4];
+  int32_t *buf = reinterpret_cast<int32_t *>(buf4);
   int carry = 0;
   for (int64_t c0 = 0; c0 < n; c0 += kTile) {
     const int m = (int)min<int64_t>(kTile, n - c0);
-    for (int i = threadIdx.x; i < m; i += kBlock) buf[i] = in[c0 + i];
+    tile_to_lds(in + c0, m, buf);
     __syncthreads();
-    int v[kItems], s = 0;
-#pragma unroll
-    for (int i = 0; i < kItems; ++i) {
-      const int k = threadIdx.x * kItems + i;
-      v[i] = k < m ? buf[k] : 0;
-      s += v[i];
-    }
     int tot = 0;
-    int run = carry + block_exclusive(s, &tot, sw);
-#pragma unroll
-    for (int i = 0; i < kItems; ++i) {
-      const int k = threadIdx.x * kItems + i;
-      if (k < m) buf[k] = run;
-      run += v[i];
-    }
+    scan_lds_tile(buf, carry, &tot, sw);
     __syncthreads();
-    for (int i = threadIdx.x; i < m; i += kBlock) out[c0 + i] = buf[i];
-    __syncthreads();  // buf is refilled by the next chunk
+    lds_to_tile(buf, m, out + c0);
+    __syncthreads();  // buf is refilled by the next tile
     carry += tot;
   }
 }
 
-// Exclusive scan of each tile, offset by its tile's scanned total.
+// Exclusive scan of each tile (in and out may alias), offset by its tile's
+// scanned total.
 __global__ __launch_bounds__(kBlock) void scan_tiles(const int32_t *in, int32_t *out, int64_t n,
                                                      const int32_t *__restrict__ part) {
   __shared__ int sw[4];
-  __shared__ int32_t buf[kTile];
+  __shared__ int4 buf4[kTile / 4];
+  int32_t *buf = reinterpret_cast<int32_t *>(buf4);
   const int64_t base = (int64_t)blockIdx.x * kTile;
   const int m = (int)min<int64_t>(kTile, n - base);
-  for (int i = threadIdx.x; i < m; i += kBlock) buf[i] = in[base + i];
+  tile_to_lds(in + base, m, buf);
   __syncthreads();
-  int v[kItems], s = 0;
-#pragma unroll
-  for (int i = 0; i < kItems; ++i) {
-    const int k = threadIdx.x * kItems + i;
-    v[i] = k < m ? buf[k] : 0;
-    s += v[i];
-  }
   int tot = 0;
-  int run = part[blockIdx.x] + block_exclusive(s, &tot, sw);
-#pragma unroll
-  for (int i = 0; i < kItems; ++i) {
-    const int k = threadIdx.x * kItems + i;
-    if (k < m) buf[k] = run;
-    run += v[i];
-  }
+  scan_lds_tile(buf, part[blockIdx.x], &tot, sw);
   __syncthreads();
-  for (int i = threadIdx.x; i < m; i += kBlock) out[base + i] = buf[i];
+  lds_to_tile(buf, m, out + base);
 }
 
 // A block's tile: its first key, key count, and where its digit counts go
