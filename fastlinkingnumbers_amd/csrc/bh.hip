@@ -699,8 +699,7 @@ __device__ __forceinline__ int wave_lds_slot(bool pred, int *counter) {
 // Learned criterion inside the level kernel: the network's biases and input
 // normalisation, and each wave's rows of features and predictions.
 struct MlpShared {
-  InferLds L;
-  InferWLds W;  // the weight fragments: the level kernel has no VGPRs to spare for them
+  InferImage img;  // weight fragments in LDS: the level kernel has no VGPRs to spare for them
   __attribute__((aligned(16))) _Float16 xs[kTravBlock / 64][64][16];
   float ys[kTravBlock / 64][64];
   unsigned int evals;  // network evaluations of the block
@@ -725,31 +724,34 @@ __device__ __forceinline__ float wave_predict(bool need, const BhGeo &A, const B
     mlp_features_t<true>(A, B, dsq, f);  // log2 features, normalised with mean2 / istd2
     f16x8 v0, v1;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) v0[j] = (_Float16)((f[j] - ms.L.mean2[j]) * ms.L.istd2[j]);
+    for (int j = 0; j < 8; ++j) v0[j] = (_Float16)((f[j] - ms.img.L.mean2[j]) * ms.img.L.istd2[j]);
 #pragma unroll
     for (int j = 0; j < 8; ++j)
-      v1[j] = (_Float16)(j < FLK_MLP_NFEAT - 8 ? (f[8 + j] - ms.L.mean2[8 + j]) * ms.L.istd2[8 + j] : 0.f);
+      v1[j] = (_Float16)(j < FLK_MLP_NFEAT - 8 ? (f[8 + j] - ms.img.L.mean2[8 + j]) * ms.img.L.istd2[8 + j] : 0.f);
     *reinterpret_cast<f16x8 *>(&ms.xs[w][pos][0]) = v0;
     *reinterpret_cast<f16x8 *>(&ms.xs[w][pos][8]) = v1;
   }
   wave_lds_sync();
-  const int cnt = __popcll(m), nt = (cnt + 15) >> 4;
+  const int cnt = __popcll(m);
   wave_evals += (unsigned int)cnt;
+  // rows past cnt are zero (their predictions are not kept); four tiles when
+  // more than two are needed, else two
   f16x8 bx[4];
   float yv[4];
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
     const int row = 16 * t + c;
-    if (t < nt && g < 2 && row < cnt) bx[t] = *reinterpret_cast<const f16x8 *>(&ms.xs[w][row][8 * g]);
+    if (g < 2 && row < cnt) bx[t] = *reinterpret_cast<const f16x8 *>(&ms.xs[w][row][8 * g]);
     else
 #pragma unroll
       for (int j = 0; j < 8; ++j) bx[t][j] = (_Float16)0.f;
   }
-  infer_tiles<4>(LdsWeights{(LdsW *)&ms.W, l}, ms.L, bx, nt, yv);
+  if (cnt > 32) infer_tiles<4>(LdsWeights{(LdsW *)&ms.img.W, l}, ms.img.L, bx, yv);
+  else infer_tiles<2>(LdsWeights{(LdsW *)&ms.img.W, l}, ms.img.L, bx, yv);
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
     const int row = 16 * t + c;
-    if (t < nt && g == 0 && row < cnt) ms.ys[w][row] = yv[t];
+    if (g == 0 && row < cnt) ms.ys[w][row] = yv[t];
   }
   wave_lds_sync();
   const float p = need ? ms.ys[w][pos] : __builtin_inff();
@@ -769,8 +771,7 @@ template <int IPT, bool MLP>
 __global__ __launch_bounds__(kTravBlock) __attribute__((amdgpu_waves_per_eu(4))) void bh_level(
     const BhItem *__restrict__ items, int64_t n, const unsigned long long *__restrict__ n_dev,
     const BhGeo *__restrict__ geo, const BhMom *__restrict__ mom, const float *__restrict__ beta2,
-    float eps, const float *__restrict__ mlp_params, const float *__restrict__ mlp_norm, float log_tol,
-    unsigned long long *__restrict__ evals,
+    float eps, const InferImage *__restrict__ mlp_img, float log_tol, unsigned long long *__restrict__ evals,
     BhItem *__restrict__ next, unsigned long long *__restrict__ next_count, int64_t next_cap,
     BhItem *__restrict__ leaves, unsigned long long *__restrict__ leaf_count, int64_t leaf_cap,
     double *__restrict__ psum, double *__restrict__ perr) {
@@ -790,11 +791,18 @@ __global__ __launch_bounds__(kTravBlock) __attribute__((amdgpu_waves_per_eu(4)))
   unsigned int wave_evals = 0;
   if constexpr (MLP) {
     if ((int64_t)blockIdx.x * kTravItems >= n) return;  // the whole block, before any barrier
-    {
-      InferRegs R;
-      load_infer(mlp_params, mlp_norm, R, ms.L);  // ends with a block barrier
-      if (threadIdx.x < 64) infer_w_store(R, ms.W);
-    }
+    // the network's LDS image, every load of a thread in flight at once
+    constexpr int kWords = (int)(sizeof(InferImage) / 16);
+    static_assert(kWords > 3 * kTravBlock && kWords <= 4 * kTravBlock, "four 16-byte words per thread");
+    const int4 *src = reinterpret_cast<const int4 *>(mlp_img) + threadIdx.x;
+    int4 *dst = reinterpret_cast<int4 *>(&ms.img) + threadIdx.x;
+    const bool w3 = 3 * kTravBlock + (int)threadIdx.x < kWords;
+    const int4 v0 = src[0], v1 = src[kTravBlock], v2 = src[2 * kTravBlock];
+    const int4 v3 = w3 ? src[3 * kTravBlock] : int4{};
+    dst[0] = v0;
+    dst[kTravBlock] = v1;
+    dst[2 * kTravBlock] = v2;
+    if (w3) dst[3 * kTravBlock] = v3;
     if (threadIdx.x == 0) ms.evals = 0;  // read after the chunk loop's barriers
     __syncthreads();
   }
@@ -1470,7 +1478,7 @@ int items_per_thread(const flk_ctx *ctx, bool use_mlp) {
 
 // The level kernel for a criterion and a number of items per thread.
 using LevelKernel = void (*)(const BhItem *, int64_t, const unsigned long long *, const BhGeo *, const BhMom *,
-                             const float *, float, const float *, const float *, float, unsigned long long *,
+                             const float *, float, const InferImage *, float, unsigned long long *,
                              BhItem *, unsigned long long *, int64_t, BhItem *, unsigned long long *, int64_t,
                              double *, double *);
 LevelKernel level_kernel(int ipt, bool use_mlp) {
@@ -1530,8 +1538,7 @@ void traverse(flk_ctx *ctx, Forest &F, const std::vector<int32_t> &ra, const std
   FLK_HIP(hipGetLastError());
   DevBuf *spare = level_buf(2 * std::min(np, kChunk));
   if (use_mlp) mlp_upload(ctx);
-  const float *mlp_params = use_mlp ? ctx->mlp.d_params.as<float>() : nullptr;
-  const float *mlp_norm = use_mlp ? ctx->mlp.d_norm.as<float>() : nullptr;
+  const InferImage *mlp_img = use_mlp ? ctx->mlp.d_image.as<InferImage>() : nullptr;
   unsigned long long visits = 0, far = 0;
   int slot_next = 0;
   DevBuf *alt = nullptr;  // second output buffer of a multi-level round
@@ -1581,7 +1588,7 @@ void traverse(flk_ctx *ctx, Forest &F, const std::vector<int32_t> &ra, const std
       const dim3 grid((unsigned)((bound + ipt * kTravBlock - 1) / (ipt * kTravBlock)));
       launch_timed(timing ? ctx->ev1 : nullptr, timing ? ctx->ev2 : nullptr, level_kernel(ipt, use_mlp), grid,
                    dim3(kTravBlock), s, in, bound, n_dev, F.geo.as<BhGeo>(), F.mom.as<BhMom>(),
-                   (const float *)B.beta2.as<float>(), eps, mlp_params, mlp_norm, log_tol, cnt + 3,
+                   (const float *)B.beta2.as<float>(), eps, mlp_img, log_tol, cnt + 3,
                    outb[d]->as<BhItem>(), qc + 2 * d, 2 * bound, B.leaves.as<BhItem>(), qc + 1, leaf_cap, d_sum,
                    d_err);
       FLK_HIP(hipGetLastError());
@@ -1709,8 +1716,7 @@ void enqueue_pass(flk_ctx *ctx, const PassShape &P, const BhItem *items0, const 
   BhScratch &B = ctx->bh;
   const Curves &cv = ctx->curves;
   BhItem *buf[2] = {B.levels[0]->as<BhItem>(), B.levels[1]->as<BhItem>()};
-  const float *mlp_params = use_mlp ? ctx->mlp.d_params.as<float>() : nullptr;
-  const float *mlp_norm = use_mlp ? ctx->mlp.d_norm.as<float>() : nullptr;
+  const InferImage *mlp_img = use_mlp ? ctx->mlp.d_image.as<InferImage>() : nullptr;
   const LevelKernel lev = level_kernel(P.ipt, use_mlp);
   auto leafk = P.ipt == 2 ? bh_leaf_pairs<2> : P.ipt == 8 ? bh_leaf_pairs<8> : bh_leaf_pairs<4>;
   // [2] leaf segment pairs, [3] network evaluations, [4 + 2d] items of level
@@ -1719,7 +1725,7 @@ void enqueue_pass(flk_ctx *ctx, const PassShape &P, const BhItem *items0, const 
     const BhItem *in = d == 0 ? items0 : buf[(d - 1) & 1];
     hipLaunchKernelGGL(lev, dim3(P.grid_lev), dim3(kTravBlock), 0, s, in, P.cap,
                        (const unsigned long long *)(cnt + 4 + 2 * d), B.geo.as<BhGeo>(), B.mom.as<BhMom>(), beta2,
-                       eps, mlp_params, mlp_norm, log_tol, cnt + 3, buf[d & 1], cnt + 4 + 2 * (d + 1), P.cap,
+                       eps, mlp_img, log_tol, cnt + 3, buf[d & 1], cnt + 4 + 2 * (d + 1), P.cap,
                        B.leaves.as<BhItem>(), cnt + 5, P.cap, d_sum, d_err);
     FLK_HIP(hipGetLastError());
   }

@@ -252,7 +252,8 @@ void flk_destroy(flk_ctx *ctx) {
                          &ctx->d_cost, &ctx->d_task_theta, &ctx->d_task_corr,
                          &ctx->d_pair_task0, &ctx->d_pair_out, &ctx->d_pair_corr,
                          &ctx->d_boxes, &ctx->d_pls, &ctx->d_count, &ctx->d_misc,
-                         &ctx->mlp.d_params, &ctx->mlp.d_norm, &ctx->mlp.d_bf16};
+                         &ctx->mlp.d_params, &ctx->mlp.d_norm, &ctx->mlp.d_bf16,
+                         &ctx->mlp.d_image};
   for (auto *b : bufs) b->release();
   ctx->cert_graph.reset();
   ctx->bh.release();

@@ -278,6 +278,7 @@ struct MlpState {
   std::vector<float> norm;    // mean[12], std[12]
   bool loaded = false;
   DevBuf d_params{true}, d_norm{true}, d_bf16;
+  DevBuf d_image{true};  // the level kernel's LDS image of the network (mlp_upload)
 };
 
 // A whole Barnes–Hut or hybrid certificate captured into one graph (bh.hip).

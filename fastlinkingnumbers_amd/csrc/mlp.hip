@@ -356,9 +356,19 @@ __global__ void adam_kernel(float *__restrict__ P, const float *__restrict__ g, 
   P[t] -= lr * mh / (sqrtf(vh) + 1e-8f);
 }
 
+// The level kernel's LDS image of the network, one wave.
+__global__ __launch_bounds__(64) void mlp_pack_image(const float *__restrict__ P, const float *__restrict__ norm,
+                                                     InferImage *__restrict__ img) {
+  InferRegs R;
+  load_infer(P, norm, R, img->L);
+  infer_w_store(R, img->W);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- host side
+// The weights from the host masters to the device, and the level kernel's
+// image of them packed from those.
 void mlp_upload(flk_ctx *ctx) {
   MlpState &M = ctx->mlp;
   if (!M.loaded) throw Error(FLK_ERR_STATE, "no MLP weights loaded (flk_mlp_set_weights)");
@@ -368,6 +378,10 @@ void mlp_upload(flk_ctx *ctx) {
                          hipMemcpyHostToDevice, ctx->stream));
   FLK_HIP(hipMemcpyAsync(M.d_norm.ptr, M.norm.data(), sizeof(float) * 2 * FLK_MLP_NFEAT,
                          hipMemcpyHostToDevice, ctx->stream));
+  M.d_image.reserve(sizeof(InferImage));
+  hipLaunchKernelGGL(mlp_pack_image, dim3(1), dim3(64), 0, ctx->stream, M.d_params.as<float>(),
+                     M.d_norm.as<float>(), M.d_image.as<InferImage>());
+  FLK_HIP(hipGetLastError());
 }
 
 void mlp_forward(flk_ctx *ctx, const float *x, int64_t n, float *y) {

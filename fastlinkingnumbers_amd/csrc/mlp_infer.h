@@ -113,6 +113,16 @@ __device__ __forceinline__ void infer_w_store(const InferRegs &R, InferWLds &W) 
   }
 }
 
+// The network as a level kernel keeps it in LDS -- weight fragments, biases
+// and input normalisation -- packed once per upload of the weights
+// (mlp_pack_image: load_infer and infer_w_store, so bit for bit what a block
+// would build itself) and copied into each block's LDS with 16-byte loads.
+struct InferImage {
+  InferWLds W;
+  InferLds L;
+};
+static_assert(sizeof(InferImage) % 16 == 0, "whole 16-byte words");
+
 // Weight sources of a tile: the fragments in VGPRs, or read from LDS where
 // each is used (volatile reads in the LDS address space: hoisted out of a
 // loop of tiles they would occupy the 56 VGPRs again).
@@ -166,15 +176,17 @@ __device__ __forceinline__ float infer_tile(const Wt &R, const InferLds &L, f16x
   return part + L.b3;
 }
 
-// Up to NT tiles of one wave through the same weight fragments: each
-// fragment is read once for all of them (read from LDS, the fragments are
-// 224 B per lane per tile, more LDS time than the tile's 12 MFMAs take) and
-// applied to every live tile in the order infer_tile applies it, so every
-// prediction is bit-identical to infer_tile's.  bx[t] is tile t's B
-// fragment; tiles t >= nt (wave-uniform) are skipped.  y[t] is the
-// prediction of tile t's sample l%16, in every lane.
+// NT tiles of one wave through the same weight fragments: each fragment is
+// read once for all of them (read from LDS, the fragments are 224 B per lane
+// per tile, more LDS time than the tile's 12 MFMAs take) and applied to every
+// tile in the order infer_tile applies it, so every prediction is
+// bit-identical to infer_tile's.  No tile is skipped: the NT independent
+// MFMAs of a step issue back to back instead of each waiting for the last,
+// and a tile of zero rows only costs its MFMAs (a sample is one output
+// column, so it never touches another's).  bx[t] is tile t's B fragment;
+// y[t] the prediction of tile t's sample l%16, in every lane.
 template <int NT, class Wt>
-__device__ __forceinline__ void infer_tiles(const Wt &R, const InferLds &L, const f16x8 *bx, int nt, float *y) {
+__device__ __forceinline__ void infer_tiles(const Wt &R, const InferLds &L, const f16x8 *bx, float *y) {
   const int l = threadIdx.x & 63, g = l >> 4;
   f16x8 bh0[NT], bh1[NT];
   {
@@ -183,24 +195,25 @@ __device__ __forceinline__ void infer_tiles(const Wt &R, const InferLds &L, cons
     for (int nb = 0; nb < 4; ++nb) {
       const f16x8 a = R.a1(nb);
       const float4 bb = *reinterpret_cast<const float4 *>(&L.b1[nb * 16 + 4 * g]);
+      f32x4 z[NT];
 #pragma unroll
       for (int t = 0; t < NT; ++t)
-        if (t < nt) {
-          const f32x4 z = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, bx[t], f32x4{bb.x, bb.y, bb.z, bb.w}, 0, 0, 0);
-          h[t][nb][0] = hswish2(z[0], z[1]);
-          h[t][nb][1] = hswish2(z[2], z[3]);
-        }
+        z[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, bx[t], f32x4{bb.x, bb.y, bb.z, bb.w}, 0, 0, 0);
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        h[t][nb][0] = hswish2(z[t][0], z[t][1]);
+        h[t][nb][1] = hswish2(z[t][2], z[t][3]);
+      }
     }
 #pragma unroll
-    for (int t = 0; t < NT; ++t)
-      if (t < nt) {
-        bh0[t] = __builtin_shufflevector(__builtin_shufflevector(h[t][0][0], h[t][0][1], 0, 1, 2, 3),
-                                         __builtin_shufflevector(h[t][1][0], h[t][1][1], 0, 1, 2, 3), 0, 1, 2, 3,
-                                         4, 5, 6, 7);
-        bh1[t] = __builtin_shufflevector(__builtin_shufflevector(h[t][2][0], h[t][2][1], 0, 1, 2, 3),
-                                         __builtin_shufflevector(h[t][3][0], h[t][3][1], 0, 1, 2, 3), 0, 1, 2, 3,
-                                         4, 5, 6, 7);
-      }
+    for (int t = 0; t < NT; ++t) {
+      bh0[t] = __builtin_shufflevector(__builtin_shufflevector(h[t][0][0], h[t][0][1], 0, 1, 2, 3),
+                                       __builtin_shufflevector(h[t][1][0], h[t][1][1], 0, 1, 2, 3), 0, 1, 2, 3, 4,
+                                       5, 6, 7);
+      bh1[t] = __builtin_shufflevector(__builtin_shufflevector(h[t][2][0], h[t][2][1], 0, 1, 2, 3),
+                                       __builtin_shufflevector(h[t][3][0], h[t][3][1], 0, 1, 2, 3), 0, 1, 2, 3, 4,
+                                       5, 6, 7);
+    }
   }
   float part[NT];
 #pragma unroll
@@ -210,23 +223,25 @@ __device__ __forceinline__ void infer_tiles(const Wt &R, const InferLds &L, cons
     const f16x8 a0 = R.a2(nb, 0), a1 = R.a2(nb, 1);
     const f16x2 w0 = R.w3(nb, 0), w1 = R.w3(nb, 1);
     const float4 bb = *reinterpret_cast<const float4 *>(&L.b2[nb * 16 + 4 * g]);
+    f32x4 z[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t)
-      if (t < nt) {
-        f32x4 z = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0, bh0[t], f32x4{bb.x, bb.y, bb.z, bb.w}, 0, 0, 0);
-        z = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1, bh1[t], z, 0, 0, 0);
-        part[t] = __builtin_amdgcn_fdot2(hswish2(z[0], z[1]), w0, part[t], false);
-        part[t] = __builtin_amdgcn_fdot2(hswish2(z[2], z[3]), w1, part[t], false);
-      }
+      z[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0, bh0[t], f32x4{bb.x, bb.y, bb.z, bb.w}, 0, 0, 0);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) z[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1, bh1[t], z[t], 0, 0, 0);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      part[t] = __builtin_amdgcn_fdot2(hswish2(z[t][0], z[t][1]), w0, part[t], false);
+      part[t] = __builtin_amdgcn_fdot2(hswish2(z[t][2], z[t][3]), w1, part[t], false);
+    }
   }
 #pragma unroll
-  for (int t = 0; t < NT; ++t)
-    if (t < nt) {
-      float p = part[t];
-      p += __shfl_xor(p, 16, 64);
-      p += __shfl_xor(p, 32, 64);
-      y[t] = p + L.b3;
-    }
+  for (int t = 0; t < NT; ++t) {
+    float p = part[t];
+    p += __shfl_xor(p, 16, 64);
+    p += __shfl_xor(p, 32, 64);
+    y[t] = p + L.b3;
+  }
 }
 
 }  // namespace
