@@ -789,8 +789,16 @@ __global__ __launch_bounds__(kTravBlock) __attribute__((amdgpu_waves_per_eu(4)))
   __shared__ TravShared sh;
   __shared__ std::conditional_t<MLP, MlpShared, NoMlpShared> ms;
   unsigned int wave_evals = 0;
+  // Items per chunk: 256 * IPT, or, for a level that one round of the grid
+  // covers, the count spread over every block in whole multiples of 256, so
+  // a small level costs its threads one or two item iterations (each a chain
+  // of dependent loads) instead of IPT.  Uniform across the grid.
+  const int64_t gsz = (int64_t)gridDim.x;
+  const int64_t csz =
+      n <= gsz * kTravItems ? min<int64_t>(kTravItems, ((n + gsz - 1) / gsz + kTravBlock - 1) / kTravBlock * kTravBlock)
+                            : kTravItems;
   if constexpr (MLP) {
-    if ((int64_t)blockIdx.x * kTravItems >= n) return;  // the whole block, before any barrier
+    if ((int64_t)blockIdx.x * csz >= n) return;  // the whole block, before any barrier
     // the network's LDS image, every load of a thread in flight at once
     constexpr int kWords = (int)(sizeof(InferImage) / 16);
     static_assert(kWords > 3 * kTravBlock && kWords <= 4 * kTravBlock, "four 16-byte words per thread");
@@ -806,14 +814,14 @@ __global__ __launch_bounds__(kTravBlock) __attribute__((amdgpu_waves_per_eu(4)))
     if (threadIdx.x == 0) ms.evals = 0;  // read after the chunk loop's barriers
     __syncthreads();
   }
-  for (int64_t chunk = blockIdx.x; chunk * kTravItems < n; chunk += gridDim.x) {
+  for (int64_t chunk = blockIdx.x; chunk * csz < n; chunk += gridDim.x) {
   if (threadIdx.x == 0) sh.nsplit = sh.nleaf = 0;
   __syncthreads();
-  const int64_t b0 = chunk * kTravItems + threadIdx.x;
+  const int64_t b0 = chunk * csz + threadIdx.x;
   int tp = -1;  // this thread's far-field pair and its running sums
   double tv = 0.0, te = 0.0;
 #pragma unroll 1
-  for (int k = 0; k < IPT; ++k) {
+  for (int k = 0; k < IPT && k * kTravBlock < csz; ++k) {
     const int64_t t = b0 + (int64_t)k * kTravBlock;
     const bool live = t < n;
     BhItem it = live ? items[t] : BhItem{0, 0, 0, 0};
