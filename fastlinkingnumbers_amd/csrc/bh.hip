@@ -922,7 +922,9 @@ __global__ __launch_bounds__(kTravBlock) __attribute__((amdgpu_waves_per_eu(4)))
 }
 
 // Exact evaluation of leaf-bucket pairs: every segment pair of the two
-// buckets with the reference leaf term, from FP32 hi/lo vertices.  Blocks of
+// buckets with the reference leaf term, from FP32 hi/lo vertices (two
+// one-segment leaves: from their node records, which hold the midpoints and
+// the edge vectors).  Blocks of
 // 256 * IPT items as in bh_level (chunks strided over the grid; with n_dev
 // the count is read on the device, n being the bound); counters[2] gets the
 // segment-pair count.
@@ -947,6 +949,14 @@ __global__ __launch_bounds__(kTravBlock) void bh_leaf_pairs(
     const BhItem it = items[t];
     const BhGeo A = geo[it.a], B = geo[it.b];
     float acc = 0.f;
+    if (is_leaf_segment(A) && is_leaf_segment(B)) {
+      // one segment each: the records already hold the midpoints (the box
+      // centres, hi/lo) and the FP32 edge vectors, so no vertex is read
+      const F3 r = f3((B.cx - A.cx) + (B.lx - A.lx), (B.cy - A.cy) + (B.ly - A.ly), (B.cz - A.cz) + (B.lz - A.lz));
+      acc = leaf_term(r, f3(__builtin_bit_cast(float, A.c1), A.pad, A.dn),
+                      f3(__builtin_bit_cast(float, B.c1), B.pad, B.dn));
+      ++npairs;
+    } else
     for (int i = A.seg0; i < A.seg0 + A.nseg; ++i) {
       const uint32_t va = order[i];
       const float4 pa = hi[va], pb = hi[va + 1], qa = lo[va], qb = lo[va + 1], ea = ed[va];
