@@ -174,3 +174,31 @@ def test_graph_recaptured_after_scratch_reallocation(engine):
     both = engine.linking_numbers([[0, 1], [2, 3]], **kw)
     eager_both = engine.linking_numbers([[0, 1], [2, 3]], graph=0, **kw)
     _same(both, eager_both)
+
+
+def test_graph_replay_uses_the_weights_set_before_it(engine):
+    """The learned level kernel copies the network from an image packed on
+    the device after every upload of the weights, outside the captured
+    graph.  A graph captured with one network and replayed after another is
+    set must give what the eager launch gives with the new one: its visits,
+    evaluations, re-runs and sums, not the old network's."""
+    curves = W.random_walk_pair(seed=2, n=20000, rg=30.0, workload=3)
+    engine.set_curves(curves)
+    P, nm = crit.load_weights()
+    kw = dict(method="barneshut", criterion="mlp", fallback_tol=0.0)
+    engine.mlp_set_weights(P, nm)
+    g1 = engine.linking_numbers([[0, 1]], **kw)
+    _same(g1, engine.linking_numbers([[0, 1]], graph=0, **kw))
+    # the output bias raised by 2: every predicted log10 error 2 larger, so
+    # fewer node pairs pass as far fields
+    P2 = P.copy()
+    P2[-1] += 2.0
+    engine.mlp_set_weights(P2, nm)
+    g2 = engine.linking_numbers([[0, 1]], **kw)
+    assert g2.stats["graph_launches"] == 1
+    e2 = engine.linking_numbers([[0, 1]], graph=0, **kw)
+    print("visits", g1.stats["bh_visits"], "->", g2.stats["bh_visits"], "retries", g2.stats["retries"])
+    _same(g2, e2)
+    assert g2.stats["bh_visits"] > g1.stats["bh_visits"]
+    engine.mlp_set_weights(P, nm)
+    _same(engine.linking_numbers([[0, 1]], **kw), g1)
