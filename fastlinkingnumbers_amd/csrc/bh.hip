@@ -2268,7 +2268,6 @@ bool bh_run_graph(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, const flk_
     auto NG = std::make_shared<CertGraph>();
     NG->key = key;
     NG->ds_idx = ds_idx;
-    if (!ds_idx.empty()) ds_capture_prepare(ctx, ds_pairs.data(), (int64_t)ds_idx.size(), prm.fp64 != 0, NG->ds);
     for (auto &idx : form_batches(ctx, pairs, bh_idx, prm)) {
       NG->batches.emplace_back();
       BatchPlan &bp = NG->batches.back();
@@ -2276,18 +2275,61 @@ bool bh_run_graph(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, const flk_
       if (!bp.device) return false;
       bp.land_pinned(s);
     }
+    if (!ds_idx.empty()) {
+      // Beside the Barnes-Hut branch the direct-sum kernel takes a share of
+      // its resident blocks, the direct sum's fraction of the two parts'
+      // estimated times alone: its segment pairs at 1.2e12 per second (FP64:
+      // 1.9e11), and 0.19 ns per Barnes-Hut tree segment for build and
+      // traversal.  Both rates are the 10,000-loop lattice's (5.6e10 direct
+      // pairs, 3.2e8 tree segments; share 0.43, 3 of the 8 blocks per CU),
+      // whose certificate took 109 ms with the parts in sequence, 97 ms as two
+      // branches at full residency, 87, 84 and 97 ms with 4, 3 and 2 blocks.
+      double share = 1.0;
+      if (!NG->batches.empty()) {
+        const std::vector<int32_t> &n = ctx->curves.n;
+        double work = 0.0, bh_segments = 0.0;
+        for (size_t k = 0; k + 1 < ds_pairs.size(); k += 2)
+          work += (double)n[ds_pairs[k]] * (double)n[ds_pairs[k + 1]];
+        for (const BatchPlan &bp : NG->batches) bh_segments += (double)bp.forest.S;
+        const double t_ds = work / (prm.fp64 ? 1.9e11 : 1.2e12), t_bh = bh_segments * 0.19e-9;
+        share = t_ds / (t_ds + t_bh);
+      }
+      ds_capture_prepare(ctx, ds_pairs.data(), (int64_t)ds_idx.size(), prm.fp64 != 0, NG->ds, share);
+    }
     FLK_HIP(hipStreamSynchronize(s));
     hipGraph_t graph = nullptr;
     FLK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     t_capturing = true;
+    // A hybrid certificate's direct-sum part (VALU-bound) and its Barnes-Hut
+    // batches (memory-bound builds and traversals) share nothing they write:
+    // the direct sum is a branch of its own, forked from the capture stream
+    // onto the context's second stream and joined before the capture ends.
+    const bool fork = !ds_idx.empty() && !NG->batches.empty();
+    bool forked = false, joined = false;
     try {
-      if (!ds_idx.empty()) ds_capture_enqueue(ctx, NG->ds, s);
+      if (fork) {
+        FLK_HIP(hipEventRecord(ctx->ev_fork, s));
+        FLK_HIP(hipStreamWaitEvent(ctx->aux, ctx->ev_fork, 0));
+        forked = true;
+        ds_capture_enqueue(ctx, NG->ds, ctx->aux);
+      } else if (!ds_idx.empty()) {
+        ds_capture_enqueue(ctx, NG->ds, s);
+      }
       for (BatchPlan &bp : NG->batches) {
         enqueue_forest(ctx, bp.forest, s);
         enqueue_traversal(ctx, bp, prm, s);
       }
+      if (forked) {
+        FLK_HIP(hipEventRecord(ctx->ev_join, ctx->aux));
+        FLK_HIP(hipStreamWaitEvent(s, ctx->ev_join, 0));
+        joined = true;
+      }
     } catch (...) {
       t_capturing = false;
+      if (forked && !joined) {  // the capture cannot end with a branch left open
+        (void)hipEventRecord(ctx->ev_join, ctx->aux);
+        (void)hipStreamWaitEvent(s, ctx->ev_join, 0);
+      }
       (void)hipStreamEndCapture(s, &graph);
       if (graph) (void)hipGraphDestroy(graph);
       throw;

@@ -471,8 +471,10 @@ struct DsPlan {
   std::vector<int64_t> prefix, task0;
 };
 
+// share: the fraction of the resident blocks per CU to launch when the grid
+// is sized from residency (a direct sum that runs beside other kernels).
 void ds_plan(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, bool fp64, const flk_tune &t, int64_t stride,
-             DsPlan &P) {
+             DsPlan &P, double share = 1.0) {
   Curves &cv = ctx->curves;
   P.fp64 = fp64;
   P.precise = cv.inexact;
@@ -489,7 +491,7 @@ void ds_plan(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, bool fp64, cons
                                  : (P.precise ? (const void *)ds_tile_f32<8, true> : (const void *)ds_tile_f32<8, false>);
       cached[slot] = resident_blocks(k);
     }
-    bpc = cached[slot];
+    bpc = std::max(1, std::min(cached[slot], (int)std::lround(cached[slot] * share)));
   }
   P.blocks = std::max(1, ctx->num_cus) * bpc;
   const int64_t nwaves = (int64_t)P.blocks * 4;
@@ -708,9 +710,9 @@ float ds_graph_run(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, bool fp64
 
 }  // namespace
 
-void ds_capture_prepare(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, bool fp64, DsGraph &G) {
+void ds_capture_prepare(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, bool fp64, DsGraph &G, double share) {
   DsPlan P;
-  ds_plan(ctx, pairs, npairs, fp64, ctx->tune, 1, P);
+  ds_plan(ctx, pairs, npairs, fp64, ctx->tune, 1, P, share);
   G.dp = std::move(P.dp);
   G.tasks = std::move(P.tasks);
   G.prefix = std::move(P.prefix);

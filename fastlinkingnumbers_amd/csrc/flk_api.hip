@@ -233,6 +233,9 @@ int32_t flk_create(int32_t device, flk_ctx **out) {
     FLK_HIP(hipEventCreate(&ctx->ev1));
     FLK_HIP(hipEventCreate(&ctx->ev2));
     FLK_HIP(hipEventCreate(&ctx->ev3));
+    FLK_HIP(hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking));
+    FLK_HIP(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+    FLK_HIP(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
     ctx->noalloc_scratch = flk::noalloc_scratch_init(device);  // once per device, this device current
   } catch (const flk::Error &e) {
     g_create_err = e.msg;
@@ -262,6 +265,9 @@ void flk_destroy(flk_ctx *ctx) {
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   if (ctx->ev2) (void)hipEventDestroy(ctx->ev2);
   if (ctx->ev3) (void)hipEventDestroy(ctx->ev3);
+  if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+  if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+  if (ctx->aux) (void)hipStreamDestroy(ctx->aux);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }

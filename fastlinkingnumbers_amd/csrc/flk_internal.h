@@ -290,6 +290,9 @@ struct flk_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+  // a captured hybrid certificate's direct-sum branch (default priority)
+  hipStream_t aux = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int num_cus = 0;
   std::string name;
   std::string err;
@@ -332,8 +335,11 @@ int64_t ds_work(flk_ctx *ctx, const int32_t *pairs, int64_t npairs);
 // the pairs into G's host arrays, reserves G's buffers and uploads the plan
 // (call it before the capture begins); ds_capture_enqueue enqueues the tile
 // kernel, the per-pair reduction and the read-back into G.h_out, which the
-// caller copies into G.out after the launch.
-void ds_capture_prepare(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, bool fp64, DsGraph &G);
+// caller copies into G.out after the launch.  share: the fraction of the
+// tile kernel's resident blocks per CU its grid takes (below 1 when it runs
+// beside other kernels; a tuned ds_blocks_per_cu takes precedence).
+void ds_capture_prepare(flk_ctx *ctx, const int32_t *pairs, int64_t npairs, bool fp64, DsGraph &G,
+                        double share = 1.0);
 void ds_capture_enqueue(flk_ctx *ctx, DsGraph &G, hipStream_t s);
 
 // Device-wide exclusive prefix sum (sort.hip); in and out may alias; temp
