@@ -66,7 +66,10 @@ typedef struct flk_params {
   int32_t graph;        /* 1 (default): a direct-sum batch, or a whole Barnes–Hut
                            certificate (its direct-sum part, tree builds and
                            traversal passes), is captured in a HIP graph and
-                           replayed when the same pairs come again;
+                           replayed when the same pairs come again; a
+                           certificate with both parts runs its direct sum as
+                           a branch of its own, on the context's second stream
+                           (default priority), beside the Barnes–Hut batches;
                            0: launched eagerly every time */
   int32_t reserved[5];
 } flk_params;
