@@ -956,19 +956,20 @@ __global__ __launch_bounds__(kTravBlock) void bh_leaf_pairs(
       acc = leaf_term(r, f3(__builtin_bit_cast(float, A.c1), A.pad, A.dn),
                       f3(__builtin_bit_cast(float, B.c1), B.pad, B.dn));
       ++npairs;
-    } else
-    for (int i = A.seg0; i < A.seg0 + A.nseg; ++i) {
-      const uint32_t va = order[i];
-      const float4 pa = hi[va], pb = hi[va + 1], qa = lo[va], qb = lo[va + 1], ea = ed[va];
-      for (int j = B.seg0; j < B.seg0 + B.nseg; ++j) {
-        const uint32_t vb = order[j];
-        const float4 ra = hi[vb], rb = hi[vb + 1], sa = lo[vb], sb = lo[vb + 1], eb = ed[vb];
-        // r = m2 - m1 = ((q0 - p0) + (q1 - p1)) / 2, each difference hi/lo exact
-        const F3 r = f3(0.5f * (((ra.x - pa.x) + (sa.x - qa.x)) + ((rb.x - pb.x) + (sb.x - qb.x))),
-                        0.5f * (((ra.y - pa.y) + (sa.y - qa.y)) + ((rb.y - pb.y) + (sb.y - qb.y))),
-                        0.5f * (((ra.z - pa.z) + (sa.z - qa.z)) + ((rb.z - pb.z) + (sb.z - qb.z))));
-        acc += leaf_term(r, f3(ea.x, ea.y, ea.z), f3(eb.x, eb.y, eb.z));
-        ++npairs;
+    } else {
+      for (int i = A.seg0; i < A.seg0 + A.nseg; ++i) {
+        const uint32_t va = order[i];
+        const float4 pa = hi[va], pb = hi[va + 1], qa = lo[va], qb = lo[va + 1], ea = ed[va];
+        for (int j = B.seg0; j < B.seg0 + B.nseg; ++j) {
+          const uint32_t vb = order[j];
+          const float4 ra = hi[vb], rb = hi[vb + 1], sa = lo[vb], sb = lo[vb + 1], eb = ed[vb];
+          // r = m2 - m1 = ((q0 - p0) + (q1 - p1)) / 2, each difference hi/lo exact
+          const F3 r = f3(0.5f * (((ra.x - pa.x) + (sa.x - qa.x)) + ((rb.x - pb.x) + (sb.x - qb.x))),
+                          0.5f * (((ra.y - pa.y) + (sa.y - qa.y)) + ((rb.y - pb.y) + (sb.y - qb.y))),
+                          0.5f * (((ra.z - pa.z) + (sa.z - qa.z)) + ((rb.z - pb.z) + (sb.z - qb.z))));
+          acc += leaf_term(r, f3(ea.x, ea.y, ea.z), f3(eb.x, eb.y, eb.z));
+          ++npairs;
+        }
       }
     }
     if (tp < 0 || tp == it.pair) {
