@@ -202,3 +202,24 @@ def test_graph_replay_uses_the_weights_set_before_it(engine):
     assert g2.stats["bh_visits"] > g1.stats["bh_visits"]
     engine.mlp_set_weights(P, nm)
     _same(engine.linking_numbers([[0, 1]], **kw), g1)
+
+
+def test_hybrid_branches_independent_of_the_direct_sum_grid(engine):
+    """A hybrid certificate's direct-sum part runs as a graph branch beside
+    the Barnes-Hut batches, on a share of its resident blocks.  The share only
+    schedules: the same certificate at full residency (a tuned block count)
+    and at one block per CU gives the same integers, visits and sums, and the
+    eager sequence gives them too."""
+    curves = W.workload(4, scale=0.02)
+    engine.set_curves(curves)
+    pairs = engine.potential_links()
+    kw = dict(method="barneshut", bh_direct_threshold=512 * 512, fallback_tol=0.0)
+    shared = engine.linking_numbers(pairs, **kw)
+    direct = (shared.flags & flk.FLK_PAIR_DIRECT) != 0
+    assert shared.stats["graph_launches"] == 1 and direct.any() and (~direct).any()
+    _same(shared, engine.linking_numbers(pairs, graph=0, **kw))
+    for bpc in (8, 1):
+        engine.set_tune(ds_blocks_per_cu=bpc)
+        g = engine.linking_numbers(pairs, **kw)
+        assert g.stats["graph_launches"] == 1
+        _same(g, shared)
