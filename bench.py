@@ -22,7 +22,11 @@ certificate (potential links, Barnes-Hut with adaptive beta, rounding).
 
 The Barnes-Hut legs (--method barneshut, and the learned-criterion comparison
 on the 2 x 8M random walks that the default run appends) also run the full
-FP32 direct sum on the same curves and report the largest difference.
+FP32 direct sum on the same curves and report the largest difference.  The
+learned-criterion comparison reports the median of five graph launches for
+the beta and learned certificates, and (`cpu_reference_bh`) the reference's
+own Barnes-Hut certificate of the same pair on the host, median of 3, beside
+the GPU's whole certificates (potential links included) timed the same way.
 --fp64-check instead runs the FP64 direct sum of the 2 x 8M pair (about 5.5
 minutes), split into 16 closed sub-loops of the first curve whose added
 chords cancel pairwise, and prints one JSON line with every estimate's error
