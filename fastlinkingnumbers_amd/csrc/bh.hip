@@ -79,8 +79,11 @@ __device__ __forceinline__ uint64_t spread16(uint64_t v) {
 // segment's first vertex.  cbox holds per curve the box corner (3 doubles)
 // and 1024 / extent per axis (0 for a flat axis).  The segmented sort keeps
 // every curve's segments in its own range, and segments sharing a cell keep
-// their order along the curve (the sort is stable).
-__global__ __launch_bounds__(256) void bh_keys(const double4 *__restrict__ pts, const int64_t *__restrict__ seg_off,
+// their order along the curve (the sort is stable).  The vertices are read as
+// VP: the FP64 coordinates (double4), or, when every stored coordinate is
+// exact in FP32, their FP32 copies (float4, half the bytes, the same values).
+template <typename VP>
+__global__ __launch_bounds__(256) void bh_keys(const VP *__restrict__ pts, const int64_t *__restrict__ seg_off,
                                                const int64_t *__restrict__ vtx_off, const double *__restrict__ cbox,
                                                const RadixTile *__restrict__ tiles,
                                                const int32_t *__restrict__ tcurve, uint32_t *__restrict__ keys,
@@ -93,8 +96,9 @@ __global__ __launch_bounds__(256) void bh_keys(const double4 *__restrict__ pts, 
   for (int o = threadIdx.x; o < tl.count; o += 256) {
     const int64_t g = (int64_t)tl.start + o;
     const int64_t v = v0 + g;
-    const double4 p = pts[v], q = pts[v + 1];
-    const double m[3] = {0.5 * (p.x + q.x), 0.5 * (p.y + q.y), 0.5 * (p.z + q.z)};
+    const VP p = pts[v], q = pts[v + 1];
+    const double m[3] = {0.5 * ((double)p.x + (double)q.x), 0.5 * ((double)p.y + (double)q.y),
+                         0.5 * ((double)p.z + (double)q.z)};
     uint32_t code = 0;
     for (int d = 0; d < 3; ++d) {
       const double u = (m[d] - lo[d]) * sc[d];
@@ -138,18 +142,20 @@ __host__ __device__ __forceinline__ int qidx(int i, int j) { return sym_idx(i, j
 // moment kernels read each node's segments as one contiguous run.  SP is
 // double2 (48 bytes per segment), or float2 (24 bytes) when every stored
 // coordinate is exact in FP32 (the curve set has no FP32 lo parts): the FP64
-// values read back are then the same bits, at half the traffic.
+// values read back are then the same bits, at half the traffic, and they are
+// gathered from the FP32 copies of the vertices (VP float4, 16 bytes a
+// vertex instead of 32).
 __device__ __forceinline__ double2 mk2(double2, double x, double y) { return make_double2(x, y); }
 __device__ __forceinline__ float2 mk2(float2, double x, double y) { return make_float2((float)x, (float)y); }
 
-template <typename SP>
+template <typename SP, typename VP>
 __global__ __launch_bounds__(256) void bh_stage_segments(const uint32_t *__restrict__ order,
-                                                         const double4 *__restrict__ pts, int64_t nseg,
+                                                         const VP *__restrict__ pts, int64_t nseg,
                                                          SP *__restrict__ sp) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nseg) return;
   const uint32_t v = order[s];
-  const double4 a = pts[v], b = pts[v + 1];
+  const VP a = pts[v], b = pts[v + 1];
   sp[3 * s] = mk2(SP{}, a.x, a.y);
   sp[3 * s + 1] = mk2(SP{}, a.z, b.x);
   sp[3 * s + 2] = mk2(SP{}, b.y, b.z);
@@ -1410,8 +1416,13 @@ void enqueue_moments(flk_ctx *ctx, const ForestPlan &P, SP *sp, hipStream_t s) {
   BhScratch &B = ctx->bh;
   const int64_t S = P.S;
   const int32_t nc = (int32_t)P.cs.size();
-  hipLaunchKernelGGL(bh_stage_segments<SP>, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, s,
-                     B.order.as<uint32_t>(), cv.dpts.as<double4>(), S, sp);
+  // FP32 staging means every coordinate is exact in FP32: read the FP32 copies
+  if constexpr (std::is_same_v<SP, float2>)
+    hipLaunchKernelGGL((bh_stage_segments<SP, float4>), dim3((unsigned)((S + 255) / 256)), dim3(256), 0, s,
+                       B.order.as<uint32_t>(), cv.hi.as<float4>(), S, sp);
+  else
+    hipLaunchKernelGGL((bh_stage_segments<SP, double4>), dim3((unsigned)((S + 255) / 256)), dim3(256), 0, s,
+                       B.order.as<uint32_t>(), cv.dpts.as<double4>(), S, sp);
   FLK_HIP(hipGetLastError());
   // level-by-level node creation; split flags alternate between two buffers
   put(B.range.ptr, P.roots.data(), P.dsrc, 3, sizeof(BhRange) * nc, s);
@@ -1458,9 +1469,14 @@ void enqueue_forest(flk_ctx *ctx, const ForestPlan &P, hipStream_t s) {
   const int64_t ntiles = (int64_t)P.tiles.size();
   put(B.rtiles.ptr, P.tiles.data(), P.dsrc, 4, sizeof(RadixTile) * ntiles, s);
   put(B.rcurve.ptr, P.tcurve.data(), P.dsrc, 5, sizeof(int32_t) * ntiles, s);
-  hipLaunchKernelGGL(bh_keys, dim3((unsigned)ntiles), dim3(256), 0, s, cv.dpts.as<double4>(), B.seg.as<int64_t>(),
-                     B.vtx.as<int64_t>(), B.box.as<double>(), B.rtiles.as<RadixTile>(), B.rcurve.as<int32_t>(),
-                     B.keys0.as<uint32_t>(), B.ids0.as<uint32_t>());
+  if (cv.inexact)
+    hipLaunchKernelGGL(bh_keys<double4>, dim3((unsigned)ntiles), dim3(256), 0, s, cv.dpts.as<double4>(),
+                       B.seg.as<int64_t>(), B.vtx.as<int64_t>(), B.box.as<double>(), B.rtiles.as<RadixTile>(),
+                       B.rcurve.as<int32_t>(), B.keys0.as<uint32_t>(), B.ids0.as<uint32_t>());
+  else  // every coordinate exact in FP32: the FP32 copies, the same values
+    hipLaunchKernelGGL(bh_keys<float4>, dim3((unsigned)ntiles), dim3(256), 0, s, cv.hi.as<float4>(),
+                       B.seg.as<int64_t>(), B.vtx.as<int64_t>(), B.box.as<double>(), B.rtiles.as<RadixTile>(),
+                       B.rcurve.as<int32_t>(), B.keys0.as<uint32_t>(), B.ids0.as<uint32_t>());
   FLK_HIP(hipGetLastError());
   radix_sort_pairs<uint32_t>(B.keys0.as<uint32_t>(), B.keys1.as<uint32_t>(), B.ids0.as<uint32_t>(),
                              B.scan.as<uint32_t>(), B.order.as<uint32_t>(), S, 30, B.tmp.ptr, s,
